@@ -1,0 +1,105 @@
+/* awfl_oracle.h -- CPU FP64 restatement of the reference ADER-WENO step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle for the
+ * HIP path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker / the timed CPU baseline.  Nothing
+ * in the product (awflcloud_amd/, include/) links or calls it.
+ *
+ * It restates, loop nest by loop nest and in the same floating-point operation
+ * order, the FP64 path of the reference:
+ *   Spatial_euler3d_cons_expl_cart_fv_Agrid.h (below "S:")  S:159-187, 274-315,
+ *     370-485, 565-594, 598-1697, 1800-2177
+ *   WenoLimiter.h:10-172, Temporal_ader.h:48-72, Profiles.h:8-73,
+ *   phys_params.h:4-10, const.h:21-29.
+ * The transform tables come from the generated awfl_tables.h; the composite
+ * operators of Spatial::init (S:274-315) are re-formed here.
+ */
+#ifndef AWFL_ORACLE_H
+#define AWFL_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXN 9
+
+typedef struct oracle_ctx oracle_ctx;
+
+/* Grid/scheme description (mirrors the YAML keys of S:197-272). */
+typedef struct {
+  int nx, ny, nz;
+  double xlen, ylen, zlen;
+  int bc_x, bc_y, bc_z;          /* 0 periodic, 1 wall (S:75-76) */
+  int weno_scalars, weno_winds;  /* S:223-224 */
+  int ord, ngll;                 /* ORD, NGLL (const.h:21-27); nAder = ngll */
+  int data_spec;                 /* background family: 1..3 const theta, 4 igw (S:78-81) */
+  int nthreads;                  /* OpenMP threads; 0 = runtime default */
+} oracle_config;
+
+oracle_ctx* oracle_create(const oracle_config* cfg);
+void oracle_destroy(oracle_ctx* c);
+int oracle_hs(const oracle_ctx* c);
+
+/* Tables as formed by Spatial::init, for tests. which: 0 wenoRecon(ord^3),
+ * 1 coefs_to_gll(ord*ngll), 2 sten_to_gll(ord*ngll), 3 derivMatrix(ngll^2),
+ * 4 idl(hs+2) then sigma, 5 gll pts ord, 6 gll wts ord, 7 gll pts ngll,
+ * 8 gll wts ngll.  Writes row-major [in][out] and returns the count. */
+int oracle_get_table(const oracle_ctx* c, int which, double* out);
+
+/* Hydrostatic background (S:370-425). which: 0 hyDensCells, 1 hyPressureCells,
+ * 2 hyThetaCells, 3 hyDensThetaCells (nz+2hs each), 4 hyDensGLL, 5 hyPressureGLL,
+ * 6 hyThetaGLL, 7 hyDensThetaGLL (nz*ngll each). */
+int oracle_get_background(const oracle_ctx* c, int which, double* out);
+
+/* dt as computeTimeStep (S:159-187): computed from the state on the first call
+ * while the cached value is <= 0, then cached. */
+double oracle_compute_time_step(oracle_ctx* c, const double* state, double cfl);
+void oracle_set_dt_cache(oracle_ctx* c, double dt);
+
+/* computeTendencies(state, tend, dt, split) (S:565-594): fills state halos,
+ * writes tend(5,nz,ny,nx) and toggles dimSwitch after the last split. */
+void oracle_compute_tendencies(oracle_ctx* c, double* state, double* tend, double dt, int split);
+/* One split's tendency for an explicit direction (0 x, 1 y, 2 z); no dimSwitch change. */
+void oracle_tendencies_dir(oracle_ctx* c, double* state, double* tend, double dt, int dir);
+/* Temporal_ader::timeStep (Temporal_ader.h:48-72), numTracers = 0. */
+void oracle_time_step(oracle_ctx* c, double* state, double dt);
+/* nsteps time steps; returns wall seconds of the stepping only. */
+double oracle_run_steps(oracle_ctx* c, double* state, double dt, int nsteps);
+
+int oracle_get_dim_switch(const oracle_ctx* c);
+void oracle_set_dim_switch(oracle_ctx* c, int v);
+
+/* ---- per-function entry points for unit tests ---- */
+/* weno::compute_weno_coefs (WenoLimiter.h:93-172) on one stencil u[ord] -> aw[ord]. */
+void oracle_weno_coefs(const oracle_ctx* c, const double* u, double* aw);
+/* reconstruct_gll_values (S:1800-1833): u[ord] -> gll[ngll]. */
+void oracle_reconstruct(const oracle_ctx* c, const double* u, int doweno, double* gll);
+/* diffTransformEulerCons{X,Y,Z} (S:1837-2091) on arrays laid out [var][kt][ii]
+ * with var = r, ru, rv, rw, rt, f1, f2, f3, f4, rt_gamma (, source for z).
+ * Level-0 entries must be set; returns all levels. For z, k_cell and the
+ * background pressure row hyPressureGLL(k,:) are taken from the context. */
+void oracle_diff_transform(const oracle_ctx* c, int dir, double* arrs, double dx, int k_cell);
+/* Riemann flux at one face (S:842-887, 1202-1245, 1583-1629), qL/qR[5] -> f[5].
+ * For z, kface selects the background pressure subtraction (S:1628-1629). */
+void oracle_riemann(const oracle_ctx* c, int dir, const double* qL, const double* qR, int kface, double* f);
+
+/* ---- initial data (S:428-485 plus the synthetic fields of the benchmarks) ---- */
+typedef struct {
+  int nbub;                      /* number of ellipsoids (<= 4) */
+  int shape[4];                  /* 0 linear cone (Profiles.h:49-58), 1 cosine (Profiles.h:60-73) */
+  double x0[4], y0[4], z0[4], xr[4], yr[4], zr[4], amp[4], pwr[4];
+  double noise_amp;              /* uniform per-cell theta' noise in [-amp, amp) */
+  uint64_t noise_seed;
+  int nmodes;                    /* band-limited theta' field evaluated at cell centres */
+  const double* modes;           /* nmodes x (kx, ky, kz, amp, phase) */
+  double u0, v0;                 /* uniform background wind (m/s), added as rho_h*u0 */
+} oracle_init_spec;
+void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* s, double* state);
+
+/* Counter-based uniform in [-1,1): SplitMix64 of (seed, index). */
+double oracle_uniform_pm1(uint64_t seed, uint64_t idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,206 @@
+"""ctypes wrapper of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's cpu_baseline leg.  The product never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OracleConfig(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xlen", C.c_double), ("ylen", C.c_double), ("zlen", C.c_double),
+                ("bc_x", C.c_int), ("bc_y", C.c_int), ("bc_z", C.c_int),
+                ("weno_scalars", C.c_int), ("weno_winds", C.c_int),
+                ("ord", C.c_int), ("ngll", C.c_int), ("data_spec", C.c_int), ("nthreads", C.c_int)]
+
+
+class OracleInitSpec(C.Structure):
+    _fields_ = [("nbub", C.c_int), ("shape", C.c_int * 4),
+                ("x0", C.c_double * 4), ("y0", C.c_double * 4), ("z0", C.c_double * 4),
+                ("xr", C.c_double * 4), ("yr", C.c_double * 4), ("zr", C.c_double * 4),
+                ("amp", C.c_double * 4), ("pwr", C.c_double * 4),
+                ("noise_amp", C.c_double), ("noise_seed", C.c_uint64),
+                ("nmodes", C.c_int), ("modes", C.POINTER(C.c_double)),
+                ("u0", C.c_double), ("v0", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: build it with `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.POINTER(C.c_double)
+        L.oracle_create.restype = P
+        L.oracle_create.argtypes = [C.POINTER(OracleConfig)]
+        L.oracle_destroy.argtypes = [P]
+        L.oracle_hs.argtypes = [P]
+        L.oracle_get_table.argtypes = [P, C.c_int, D]
+        L.oracle_get_background.argtypes = [P, C.c_int, D]
+        L.oracle_compute_time_step.restype = C.c_double
+        L.oracle_compute_time_step.argtypes = [P, D, C.c_double]
+        L.oracle_set_dt_cache.argtypes = [P, C.c_double]
+        L.oracle_compute_tendencies.argtypes = [P, D, D, C.c_double, C.c_int]
+        L.oracle_tendencies_dir.argtypes = [P, D, D, C.c_double, C.c_int]
+        L.oracle_time_step.argtypes = [P, D, C.c_double]
+        L.oracle_run_steps.restype = C.c_double
+        L.oracle_run_steps.argtypes = [P, D, C.c_double, C.c_int]
+        L.oracle_get_dim_switch.argtypes = [P]
+        L.oracle_set_dim_switch.argtypes = [P, C.c_int]
+        L.oracle_weno_coefs.argtypes = [P, D, D]
+        L.oracle_reconstruct.argtypes = [P, D, C.c_int, D]
+        L.oracle_diff_transform.argtypes = [P, C.c_int, D, C.c_double, C.c_int]
+        L.oracle_riemann.argtypes = [P, C.c_int, D, D, C.c_int, D]
+        L.oracle_init_state.argtypes = [P, C.POINTER(OracleInitSpec), D]
+        L.oracle_uniform_pm1.restype = C.c_double
+        L.oracle_uniform_pm1.argtypes = [C.c_uint64, C.c_uint64]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_spec(bubbles=(), noise_amp=0.0, noise_seed=0, modes=None, u0=0.0, v0=0.0):
+    """bubbles: iterable of dicts(shape, x0, y0, z0, xr, yr, zr, amp, pwr)."""
+    s = OracleInitSpec()
+    bubbles = list(bubbles)
+    assert len(bubbles) <= 4
+    s.nbub = len(bubbles)
+    for b, d in enumerate(bubbles):
+        s.shape[b] = d.get("shape", 1)
+        for key in ("x0", "y0", "z0", "xr", "yr", "zr", "amp"):
+            getattr(s, key)[b] = d[key]
+        s.pwr[b] = d.get("pwr", 1.0)
+    s.noise_amp = noise_amp
+    s.noise_seed = noise_seed
+    s._modes_keep = None
+    if modes is not None and len(modes):
+        m = np.ascontiguousarray(np.asarray(modes, dtype=np.float64).reshape(-1, 5))
+        s._modes_keep = m
+        s.nmodes = m.shape[0]
+        s.modes = _dp(m)
+    s.u0, s.v0 = u0, v0
+    return s
+
+
+class Oracle:
+    """One reference Spatial + Temporal instance on the CPU."""
+
+    def __init__(self, nx, ny, nz, xlen=20000.0, ylen=20000.0, zlen=10000.0, bc=(0, 0, 1),
+                 ord=5, ngll=3, weno_scalars=True, weno_winds=True, data_spec=1, nthreads=0):
+        cfg = OracleConfig(nx, ny, nz, xlen, ylen, zlen, bc[0], bc[1], bc[2], int(weno_scalars),
+                           int(weno_winds), ord, ngll, data_spec, nthreads)
+        self.L = lib()
+        self.h = self.L.oracle_create(C.byref(cfg))
+        if not self.h:
+            raise ValueError("oracle_create rejected the configuration")
+        self.nx, self.ny, self.nz, self.ord, self.ngll = nx, ny, nz, ord, ngll
+        self.hs = (ord - 1) // 2
+        self.dx, self.dy, self.dz = xlen / nx, ylen / ny, zlen / nz
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    # --- layout helpers ---
+    def state_shape(self):
+        hs = self.hs
+        return (5, self.nz + 2 * hs, self.ny + 2 * hs, self.nx + 2 * hs)
+
+    def tend_shape(self):
+        return (5, self.nz, self.ny, self.nx)
+
+    def new_state(self):
+        return np.zeros(self.state_shape())
+
+    def interior(self, st):
+        hs = self.hs
+        return st[:, hs:hs + self.nz, hs:hs + self.ny, hs:hs + self.nx]
+
+    # --- wrapped calls ---
+    def table(self, which):
+        out = np.zeros(1024)
+        n = self.L.oracle_get_table(self.h, which, _dp(out))
+        return out[:n].copy()
+
+    def background(self, which):
+        out = np.zeros(self.nz * max(self.ngll, 1) + 2 * self.hs + self.nz)
+        n = self.L.oracle_get_background(self.h, which, _dp(out))
+        return out[:n].copy()
+
+    def init_state(self, spec):
+        st = self.new_state()
+        self.L.oracle_init_state(self.h, C.byref(spec), _dp(st))
+        return st
+
+    def compute_time_step(self, state, cfl=0.8):
+        return self.L.oracle_compute_time_step(self.h, _dp(state), cfl)
+
+    def set_dt_cache(self, dt):
+        self.L.oracle_set_dt_cache(self.h, dt)
+
+    def compute_tendencies(self, state, dt, split):
+        tend = np.zeros(self.tend_shape())
+        self.L.oracle_compute_tendencies(self.h, _dp(state), _dp(tend), dt, split)
+        return tend
+
+    def tendencies_dir(self, state, dt, direction):
+        tend = np.zeros(self.tend_shape())
+        self.L.oracle_tendencies_dir(self.h, _dp(state), _dp(tend), dt, direction)
+        return tend
+
+    def time_step(self, state, dt):
+        self.L.oracle_time_step(self.h, _dp(state), dt)
+
+    def run_steps(self, state, dt, nsteps):
+        return self.L.oracle_run_steps(self.h, _dp(state), dt, nsteps)
+
+    @property
+    def dim_switch(self):
+        return self.L.oracle_get_dim_switch(self.h)
+
+    @dim_switch.setter
+    def dim_switch(self, v):
+        self.L.oracle_set_dim_switch(self.h, int(v))
+
+    def weno_coefs(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        aw = np.zeros(self.ord)
+        self.L.oracle_weno_coefs(self.h, _dp(u), _dp(aw))
+        return aw
+
+    def reconstruct(self, u, doweno=True):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        g = np.zeros(self.ngll)
+        self.L.oracle_reconstruct(self.h, _dp(u), int(doweno), _dp(g))
+        return g
+
+    def diff_transform(self, direction, arrs, dx, k_cell=0):
+        a = np.ascontiguousarray(arrs, dtype=np.float64).copy()
+        self.L.oracle_diff_transform(self.h, direction, _dp(a), dx, k_cell)
+        return a
+
+    def riemann(self, direction, qL, qR, kface=0):
+        qL = np.ascontiguousarray(qL, dtype=np.float64)
+        qR = np.ascontiguousarray(qR, dtype=np.float64)
+        f = np.zeros(5)
+        self.L.oracle_riemann(self.h, direction, _dp(qL), _dp(qR), kface, _dp(f))
+        return f
+
+
+def uniform_pm1(seed, idx):
+    return lib().oracle_uniform_pm1(seed, idx)
