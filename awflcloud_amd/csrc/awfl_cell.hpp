@@ -1,0 +1,347 @@
+// awfl_cell.hpp -- per-cell and per-face numerics of one split sweep, in
+// registers: WENO reconstruction to GLL points, ADER differential transform
+// of the Euler fluxes, time averaging of the cell-edge states, and the
+// upwind characteristic Riemann flux.
+//
+// Every routine restates the reference's formula in the reference's operation
+// order (the FP64 CPU oracle follows the same order); the only arithmetic
+// differences are FMA contraction and the device's pow/sqrt.
+//   WENO ..................... WenoLimiter.h:12-25, 93-172
+//   reconstruction ........... Spatial_euler3d_cons_expl_cart_fv_Agrid.h (S:) 1800-1833
+//   level-0 products ......... S:704-715, 1067-1078, 1428-1440
+//   differential transform ... S:1837-1916 (x), 1920-2000 (y), 2004-2091 (z)
+//   time averages ............ S:2130-2177
+//   Riemann fluxes ........... S:842-887 (x), 1202-1245 (y), 1583-1629 (z)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "awfl_tables.hpp"
+
+namespace awfl {
+
+// phys_params.h:4-10
+constexpr double GRAV = 9.81;
+constexpr double C0 = 27.5629410929725921310572974482;
+constexpr double GAMMA = 1.40027894002789400278940027894;
+constexpr double WENO_EPS = 1.0e-20;     // WenoLimiter.h:22, 100
+
+enum { idR = 0, idU = 1, idV = 2, idW = 3, idT = 4 };
+
+// ---------------------------------------------------------------------------
+// truncated total-variation indicator: the generated quadratic form of
+// coefs_to_tv<N>, terms summed left to right
+// ---------------------------------------------------------------------------
+template <int N, int M>
+__device__ __forceinline__ double tv_eval(const double (&a)[M]) {
+  double r = 0.0;
+  sfor<Tv<N>::n>([&](auto t) {
+    constexpr int j = Tv<N>::j[t];
+    constexpr int k = Tv<N>::k[t];
+    constexpr double c = Tv<N>::c[t];
+    double term;
+    if constexpr (j == k) term = c * (a[j] * a[j]);
+    else term = c * a[j] * a[k];
+    if constexpr (t == 0) r = term;
+    else r += term;
+  });
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// weno::compute_weno_coefs: ord cell averages u -> ord polynomial coefficients
+// ---------------------------------------------------------------------------
+template <int ORD>
+__device__ __forceinline__ void weno_coefs(const double (&u)[ORD], double (&aw)[ORD]) {
+  using W = Weno<ORD>;
+  constexpr int hs = W::hs;
+  double a[hs + 2][ORD];
+  // hs+1 degree-hs candidates on sub-stencils i..i+hs (WenoLimiter.h:110-116)
+  sfor<hs + 1>([&](auto i) {
+    sfor<hs + 1>([&](auto ii) {
+      double t = -0.0;
+      sfor<hs + 1>([&](auto s) {
+        constexpr double cf = W::recon[i][s][ii];
+        if constexpr (cf != 0.0) t += cf * u[i + s];
+      });
+      a[i][ii] = t;
+    });
+  });
+  // order-ORD polynomial on the full stencil (:117-121)
+  sfor<ORD>([&](auto ii) {
+    double t = -0.0;
+    sfor<ORD>([&](auto s) {
+      constexpr double cf = W::recon[hs + 1][s][ii];
+      if constexpr (cf != 0.0) t += cf * u[s];
+    });
+    a[hs + 1][ii] = t;
+  });
+  // "bridge" polynomial (:123-131)
+  sfor<hs + 1>([&](auto i) {
+    sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= W::idl[i] * a[i][ii]; });
+  });
+  sfor<ORD>([&](auto ii) { a[hs + 1][ii] /= W::idl[hs + 1]; });
+  // total variation of every candidate (:133-143), sigma blend (:145-151)
+  double tv[hs + 2];
+  sfor<hs + 1>([&](auto i) { tv[i] = tv_eval<hs + 1>(a[i]); });
+  tv[hs + 1] = tv_eval<ORD>(a[hs + 1]);
+  double lo_avg = 0.0;
+  sfor<hs + 1>([&](auto i) { lo_avg += tv[i]; });
+  lo_avg /= (hs + 1);
+  tv[hs + 1] = lo_avg + (tv[hs + 1] - lo_avg) * W::sigma;
+  // weights ~ idl / TV^2, convexify, map, convexify (:153-161)
+  double w[hs + 2];
+  sfor<hs + 2>([&](auto i) { w[i] = W::idl[i] / (tv[i] * tv[i] + WENO_EPS); });
+  {
+    double sum = 0.0;
+    sfor<hs + 2>([&](auto i) { sum += w[i]; });
+    sfor<hs + 2>([&](auto i) { w[i] /= (sum + WENO_EPS); });
+  }
+  sfor<hs + 2>([&](auto i) {
+    constexpr double d = W::idl[i];
+    w[i] = w[i] * (d + d * d - 3.0 * d * w[i] + w[i] * w[i]) / (d * d + w[i] * (1.0 - 2.0 * d));
+  });
+  {
+    double sum = 0.0;
+    sfor<hs + 2>([&](auto i) { sum += w[i]; });
+    sfor<hs + 2>([&](auto i) { w[i] /= (sum + WENO_EPS); });
+  }
+  // weighted sum (:163-171); candidates have no coefficients above hs
+  sfor<ORD>([&](auto ii) {
+    double t = -0.0;
+    sfor<hs + 2>([&](auto i) {
+      if constexpr (i == hs + 1 || ii <= hs) t += w[i] * a[i][ii];
+    });
+    aw[ii] = t;
+  });
+}
+
+// reconstruct_gll_values (S:1800-1833): ord stencil -> NGLL point values
+template <int ORD, int NGLL>
+__device__ __forceinline__ void reconstruct(const double (&u)[ORD], bool doweno, double (&g)[NGLL]) {
+  using P = Proj<ORD, NGLL>;
+  if (doweno) {
+    double aw[ORD];
+    weno_coefs<ORD>(u, aw);
+    sfor<NGLL>([&](auto ii) {
+      double t = -0.0;
+      sfor<ORD>([&](auto s) {
+        constexpr double cf = P::c2g[s][ii];
+        if constexpr (cf != 0.0) t += cf * aw[s];
+      });
+      g[ii] = t;
+    });
+  } else {
+    sfor<NGLL>([&](auto ii) {
+      double t = -0.0;
+      sfor<ORD>([&](auto s) {
+        constexpr double cf = P::s2g[s][ii];
+        if constexpr (cf != 0.0) t += cf * u[s];
+      });
+      g[ii] = t;
+    });
+  }
+}
+
+// Background seen by one cell of a sweep.  X and Y add the cell averages
+// hyDensCells/hyDensThetaCells (S:680, 697, 1043, 1060); Z adds the GLL point
+// values and subtracts the background pressure (S:1400, 1421, 2042).
+template <int NGLL>
+struct CellBg {
+  double r, rt;            // x/y: cell averages of the background
+  double rG[NGLL];         // z: hyDensGLL(k, :)
+  double rtG[NGLL];        // z: hyDensThetaGLL(k, :)
+  double pG[NGLL];         // z: hyPressureGLL(k, :)
+  bool wall_bot, wall_top; // z: solid wall and k == 0 / k == nz-1
+};
+
+// ---------------------------------------------------------------------------
+// One cell of a DIR sweep (0 x, 1 y, 2 z): stencils of the 5 state variables
+// along the sweep axis -> time-averaged left/right edge states (and, for z,
+// the time- and cell-averaged gravity source of rho*w).
+// ---------------------------------------------------------------------------
+template <int ORD, int NGLL, int DIR>
+__device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s,
+                                           bool weno_w, double dt, double dsw, double (&eL)[5], double (&eR)[5],
+                                           double& src_avg) {
+  constexpr int NA = NGLL;             // nAder = ngll (Temporal_ader.h:8)
+  constexpr int NM = 1 + DIR;          // momentum normal to the sweep
+  double q[5][NA][NGLL];               // r, ru, rv, rw, rt at every level
+  double f[4][NA][NGLL];               // q_nm*var_l/r for l = 1..4
+  double g[NA][NGLL];                  // (r t)^gamma, levels >= 1 carried doubled (S:1904-1915)
+  double src[NA][NGLL];                // z gravity source
+
+  // level 0: reconstruction + background
+  sfor<5>([&](auto l) {
+    double gv[NGLL];
+    reconstruct<ORD, NGLL>(st[l], (l == idR || l == idT) ? weno_s : weno_w, gv);
+    sfor<NGLL>([&](auto ii) { q[l][0][ii] = gv[ii]; });
+  });
+  sfor<NGLL>([&](auto ii) {
+    if constexpr (DIR == 2) {
+      q[idR][0][ii] += bg.rG[ii];
+      q[idT][0][ii] += bg.rtG[ii];
+    } else {
+      q[idR][0][ii] += bg.r;
+      q[idT][0][ii] += bg.rt;
+    }
+  });
+  if constexpr (DIR == 2) {            // S:1413-1416
+    if (bg.wall_top) q[idW][0][NGLL - 1] = 0.0;
+    if (bg.wall_bot) q[idW][0][0] = 0.0;
+  }
+  double sv = 0.0;
+  sfor<NGLL>([&](auto ii) {
+    double r = q[idR][0][ii];
+    double vel[5];
+    vel[1] = q[1][0][ii] / r;
+    vel[2] = q[2][0][ii] / r;
+    vel[3] = q[3][0][ii] / r;
+    vel[4] = q[4][0][ii] / r;
+    sfor<4>([&](auto m) { f[m][0][ii] = r * vel[NM] * vel[m + 1]; });
+    g[0][ii] = pow(r * vel[4], GAMMA);
+    if constexpr (DIR == 2) sv = -(q[idR][0][ii] - bg.rG[ii]) * GRAV;
+  });
+  if constexpr (DIR == 2) {
+    // S:1439 assigns a scalar to the whole SArray: every entry holds the value
+    // of the last GLL point
+    sfor<NA>([&](auto kt) { sfor<NGLL>([&](auto ii) { src[kt][ii] = sv; }); });
+  }
+
+  // differential transform: levels 1..NA-1
+  sfor<NA - 1>([&](auto kt) {
+    constexpr double kt1 = kt + 1.0;
+    sfor<NGLL>([&](auto ii) {
+      double d[5];
+      sfor<5>([&](auto l) { d[l] = -0.0; });
+      sfor<NGLL>([&](auto s) {
+        constexpr double D = Deriv<NGLL>::m[s][ii];
+        if constexpr (D == 0.0) return;
+        d[0] += D * q[NM][kt][s];
+        sfor<4>([&](auto m) {
+          constexpr int l = m + 1;
+          if constexpr (l == NM) {
+            if constexpr (kt == 0) {
+              if constexpr (DIR == 2) d[l] += D * (f[m][kt][s] + C0 * g[kt][s] - bg.pG[s]);
+              else d[l] += D * (f[m][kt][s] + C0 * g[kt][s]);
+            } else {
+              d[l] += D * (f[m][kt][s] + C0 * g[kt][s] / 2);
+            }
+          } else {
+            d[l] += D * f[m][kt][s];
+          }
+        });
+      });
+      sfor<5>([&](auto l) {
+        if constexpr (DIR == 2 && l == idW) q[l][kt + 1][ii] = -d[l] / dsw / kt1 + src[kt][ii] / kt1;
+        else q[l][kt + 1][ii] = -d[l] / dsw / kt1;
+      });
+      if constexpr (DIR == 2) {        // S:2051-2055
+        if (bg.wall_top) q[idW][kt + 1][NGLL - 1] = 0.0;
+        if (bg.wall_bot) q[idW][kt + 1][0] = 0.0;
+        src[kt + 1][ii] = -q[idR][kt + 1][ii] * GRAV;
+      }
+    });
+    if constexpr (kt + 2 < NA) {       // products at level kt+1 are used only below the last level
+      sfor<NGLL>([&](auto ii) {
+        sfor<4>([&](auto m) {
+          constexpr int l = m + 1;
+          double tot = 0.0;
+          sfor<kt + 2>([&](auto ir) {
+            if constexpr (ir == 0) tot += q[NM][ir][ii] * q[l][kt + 1 - ir][ii];   // f(kt+1) is still 0
+            else tot += q[NM][ir][ii] * q[l][kt + 1 - ir][ii] - q[idR][ir][ii] * f[m][kt + 1 - ir][ii];
+          });
+          f[m][kt + 1][ii] = tot / q[idR][0][ii];
+        });
+        double tg = 0.0;
+        sfor<kt + 1>([&](auto ir) {
+          constexpr double wgt = kt1 - ir;
+          if constexpr (ir == 0)          // g(kt+1) is still 0
+            tg += wgt * (GAMMA * g[ir][ii] * q[idT][kt + 1 - ir][ii]);
+          else
+            tg += wgt * (GAMMA * g[ir][ii] * q[idT][kt + 1 - ir][ii] - q[idT][ir][ii] * g[kt + 1 - ir][ii]);
+        });
+        g[kt + 1][ii] = (GAMMA * g[0][ii] * q[idT][kt + 1][ii] + tg / kt1) / q[idT][0][ii];
+      });
+    }
+  });
+
+  // time averages of the edge values (S:2142-2177)
+  sfor<5>([&](auto l) {
+    double a0 = q[l][0][0], a1 = q[l][0][NGLL - 1];
+    double dtmult = dt;
+    sfor<NA - 1>([&](auto km) {
+      constexpr int kt = km + 1;
+      a0 += q[l][kt][0] * dtmult / (kt + 1.0);
+      a1 += q[l][kt][NGLL - 1] * dtmult / (kt + 1.0);
+      dtmult *= dt;
+    });
+    eL[l] = a0;
+    eR[l] = a1;
+  });
+  if constexpr (DIR == 2) {            // S:1462, 1489-1492
+    double dtmult = dt;
+    sfor<NA - 1>([&](auto km) {
+      constexpr int kt = km + 1;
+      sfor<NGLL>([&](auto ii) { src[0][ii] += src[kt][ii] * dtmult / (kt + 1.0); });
+      dtmult *= dt;
+    });
+    double sa = 0.0;
+    sfor<NGLL>([&](auto ii) { sa += src[0][ii] * Gll<NGLL>::wts[ii]; });
+    src_avg = sa;
+  } else {
+    src_avg = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Upwind characteristic flux at one face.  X and Z average primitive
+// variables; Y averages the conservative variables (S:1210-1214).  The
+// normal velocity test is strict (u > 0).
+// ---------------------------------------------------------------------------
+template <int DIR>
+__device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], double (&F)[5]) {
+  constexpr int NM = 1 + DIR;
+  // tangential momentum indices in the order the reference uses them
+  constexpr int T1 = DIR == 0 ? 2 : 1;
+  constexpr int T2 = DIR == 2 ? 2 : 3;
+  double r = 0.5 * (L[0] + R[0]);
+  double un, ut1, ut2, t;
+  if constexpr (DIR == 1) {
+    un = 0.5 * (L[NM] + R[NM]);
+    ut1 = 0.5 * (L[T1] + R[T1]);
+    ut2 = 0.5 * (L[T2] + R[T2]);
+    t = 0.5 * (L[4] + R[4]);
+  } else {
+    un = 0.5 * (L[NM] / L[0] + R[NM] / R[0]);
+    ut1 = 0.5 * (L[T1] / L[0] + R[T1] / R[0]);
+    ut2 = 0.5 * (L[T2] / L[0] + R[T2] / R[0]);
+    t = 0.5 * (L[4] / L[0] + R[4] / R[0]);
+  }
+  double p = C0 * pow(r * t, GAMMA);
+  double cs2 = GAMMA * p / r;
+  double cs = sqrt(cs2);
+  double w1, w2, w3;
+  if (un > 0) {
+    w1 = L[0] - L[4] / t;
+    w2 = L[T1] - ut1 * L[4] / t;
+    w3 = L[T2] - ut2 * L[4] / t;
+  } else {
+    w1 = R[0] - R[4] / t;
+    w2 = R[T1] - ut1 * R[4] / t;
+    w3 = R[T2] - ut2 * R[4] / t;
+  }
+  double w5 = un * R[0] / (2 * cs) - R[NM] / (2 * cs) + R[4] / (2 * t);
+  double w6 = -un * L[0] / (2 * cs) + L[NM] / (2 * cs) + L[4] / (2 * t);
+  double q1 = w1 + w5 + w6;
+  double qn = un * w1 + (un - cs) * w5 + (un + cs) * w6;
+  double qt1 = w2 + ut1 * w5 + ut1 * w6;
+  double qt2 = w3 + ut2 * w5 + ut2 * w6;
+  double q5 = t * w5 + t * w6;
+  F[0] = qn;
+  F[NM] = qn * qn / q1 + C0 * pow(q5, GAMMA);
+  F[T1] = qn * qt1 / q1;
+  F[T2] = qn * qt2 / q1;
+  F[4] = qn * q5 / q1;
+}
+
+}  // namespace awfl

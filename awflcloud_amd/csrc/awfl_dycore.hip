@@ -1,0 +1,904 @@
+// awfl_dycore.hip -- fused ADER-WENO split sweeps for gfx950 and the C ABI of
+// include/awfl_dycore.h.
+//
+// Design (one GPU, FP64):
+//  * One kernel per split sweep.  It reads the state, generates periodic or
+//    solid-wall boundary values on the fly (no halo pass), reconstructs every
+//    cell, evolves it with the ADER differential transform, time-averages the
+//    edge states, computes the upwind flux at every face and writes
+//    out = in + dt * tendency (or the tendency itself).  No intermediate
+//    (edge states, fluxes) touches HBM.
+//  * x sweep (contiguous axis): one workgroup per x-line; the line and its
+//    boundary cells are staged in LDS with coalesced loads, edge states and
+//    face fluxes are exchanged through LDS.
+//  * y and z sweeps (strided axes): one lane per x-column walks a segment of
+//    its column along the sweep axis with a register window of ord cells, so
+//    every wavefront load/store is 64 consecutive doubles along x and no LDS
+//    or barrier is needed.
+//  * Sweeps run out of place; a fused time step ping-pongs through two
+//    context-owned buffers and ends in the caller's state buffer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/awfl_dycore.h"
+#include "awfl_cell.hpp"
+
+using namespace awfl;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// grid description passed to kernels by value
+// ---------------------------------------------------------------------------
+struct Grid {
+  int nx, ny, nz, hs;
+  int64_t sy, sz, sl;            // strides of the padded state (x stride 1)
+  int bc[3];
+  int weno_s, weno_w, sim2d;
+  double dx, dy, dz;
+  const double* hyDensCells;     // nz+2hs
+  const double* hyDensThetaCells;
+  const double* hyDensGLL;       // nz*ngll
+  const double* hyDensThetaGLL;
+  const double* hyPressureGLL;
+};
+
+__device__ __forceinline__ int64_t tend_index(const Grid& G, int l, int k, int j, int i) {
+  return (((int64_t)l * G.nz + k) * G.ny + j) * G.nx + i;
+}
+
+// ---------------------------------------------------------------------------
+// x sweep: one workgroup per x-line
+// ---------------------------------------------------------------------------
+template <int ORD, int NGLL, bool TEND>
+__global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, double* __restrict__ out, Grid G,
+                                                 double dt) {
+  constexpr int hs = (ORD - 1) / 2;
+  extern __shared__ double sm[];
+  const int nx = G.nx;
+  const int W = nx + 2 * hs;
+  const int j = blockIdx.x % G.ny;
+  const int k = blockIdx.x / G.ny;
+  double* s_st = sm;              // [5][W]   state of the line with boundary cells
+  double* s_L = sm + 5 * W;       // [5][nx]  left edge states
+  double* s_R = s_L + 5 * nx;     // [5][nx]  right edge states
+  double* s_F = sm;               // [5][nx+1] face fluxes, reuses s_st after the edges are formed
+  const int64_t row = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs;
+  const bool per = G.bc[0] == AWFL_BC_PERIODIC;
+
+  for (int c = threadIdx.x; c < W; c += blockDim.x) {
+    int ci = c - hs, src;
+    bool zn = false;
+    if (per) {
+      src = ci % nx;
+      if (src < 0) src += nx;
+    } else if (ci < 0) {
+      src = 0; zn = true;
+    } else if (ci >= nx) {
+      src = nx - 1; zn = true;
+    } else {
+      src = ci;
+    }
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      double v = in[l * G.sl + row + src];
+      s_st[l * W + c] = (zn && l == idU) ? 0.0 : v;
+    }
+  }
+  __syncthreads();
+
+  CellBg<NGLL> bg;
+  bg.r = G.hyDensCells[hs + k];
+  bg.rt = G.hyDensThetaCells[hs + k];
+  bg.wall_bot = bg.wall_top = false;
+  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
+    double st[5][ORD];
+    sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { st[l][s] = s_st[l * W + c + s]; }); });
+    double eL[5], eR[5], sa;
+    cell_edges<ORD, NGLL, 0>(st, bg, G.weno_s, G.weno_w, dt, G.dx, eL, eR, sa);
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      s_L[l * nx + c] = eL[l];
+      s_R[l * nx + c] = eR[l];
+    }
+  }
+  __syncthreads();
+
+  for (int f = threadIdx.x; f <= nx; f += blockDim.x) {
+    int a, b;                 // qL from the right edge of cell a, qR from the left edge of cell b
+    bool aL = false, bR = false;
+    if (per) {
+      a = f == 0 ? nx - 1 : f - 1;
+      b = f == nx ? 0 : f;
+    } else if (f == 0) {      // wall: both sides from the first cell's left edge (S:822)
+      a = 0; b = 0; aL = true;
+    } else if (f == nx) {     // wall: both sides from the last cell's right edge (S:823)
+      a = nx - 1; b = nx - 1; bR = true;
+    } else {
+      a = f - 1; b = f;
+    }
+    double qL[5], qR[5], F[5];
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      qL[l] = aL ? s_L[l * nx + a] : s_R[l * nx + a];
+      qR[l] = bR ? s_R[l * nx + b] : s_L[l * nx + b];
+    }
+    riemann<0>(qL, qR, F);
+#pragma unroll
+    for (int l = 0; l < 5; l++) s_F[l * (nx + 1) + f] = F[l];
+  }
+  __syncthreads();
+
+  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      double td = -(s_F[l * (nx + 1) + c + 1] - s_F[l * (nx + 1) + c]) / G.dx;
+      if (G.sim2d && l == idV) td = 0.0;
+      if constexpr (TEND) out[tend_index(G, l, k, j, c)] = td;
+      else out[l * G.sl + row + c] = in[l * G.sl + row + c] + dt * td;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweeps: one lane per x-column walking a segment of its column
+// ---------------------------------------------------------------------------
+template <int ORD, int NGLL, int DIR, bool TEND>
+__global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in, double* __restrict__ out, Grid G,
+                                                  double dt, int seglen) {
+  constexpr int hs = (ORD - 1) / 2;
+  constexpr int NM = 1 + DIR;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nx) return;              // no barrier in this kernel
+  const int o = blockIdx.y;           // y sweep: k; z sweep: j
+  const int n = DIR == 1 ? G.ny : G.nz;
+  const int64_t sa = DIR == 1 ? G.sy : G.sz;
+  const double dsw = DIR == 1 ? G.dy : G.dz;
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const int64_t base = DIR == 1 ? (int64_t)(hs + o) * G.sz + (int64_t)hs * G.sy + hs + i
+                                : (int64_t)hs * G.sz + (int64_t)(hs + o) * G.sy + hs + i;
+  const int s0 = blockIdx.z * seglen;
+  const int s1 = min(n, s0 + seglen);
+
+  auto ld = [&](int l, int a) -> double {
+    int src;
+    bool zn = false;
+    if (per) {
+      src = a % n;
+      if (src < 0) src += n;
+    } else if (a < 0) {
+      src = 0; zn = true;
+    } else if (a >= n) {
+      src = n - 1; zn = true;
+    } else {
+      src = a;
+    }
+    double v = in[l * G.sl + base + (int64_t)src * sa];
+    return (zn && l == NM) ? 0.0 : v;
+  };
+  auto bg_of = [&](int kc, CellBg<NGLL>& bg) {
+    if constexpr (DIR == 1) {
+      bg.r = G.hyDensCells[hs + o];
+      bg.rt = G.hyDensThetaCells[hs + o];
+      bg.wall_bot = bg.wall_top = false;
+    } else {
+      sfor<NGLL>([&](auto ii) {
+        bg.rG[ii] = G.hyDensGLL[kc * NGLL + ii];
+        bg.rtG[ii] = G.hyDensThetaGLL[kc * NGLL + ii];
+        bg.pG[ii] = G.hyPressureGLL[kc * NGLL + ii];
+      });
+      bg.wall_bot = !per && kc == 0;
+      bg.wall_top = !per && kc == n - 1;
+    }
+  };
+  auto face_bg = [&](int f, double (&F)[5]) {   // S:1628-1629
+    if constexpr (DIR == 2) {
+      if (f < G.nz) F[idW] -= G.hyPressureGLL[f * NGLL + 0];
+      else F[idW] -= G.hyPressureGLL[(G.nz - 1) * NGLL + NGLL - 1];
+    }
+  };
+  auto emit = [&](int c, const double (&F0)[5], const double (&F1)[5], double src, const double (&cell)[5]) {
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      double td = -(F1[l] - F0[l]) / dsw;
+      if (DIR == 2 && l == idW) td = src + td;
+      if (DIR == 2 && G.sim2d && l == idV) td = 0.0;
+      if constexpr (TEND) {
+        if constexpr (DIR == 1) out[tend_index(G, l, o, c, i)] = td;
+        else out[tend_index(G, l, c, o, i)] = td;
+      } else {
+        out[l * G.sl + base + (int64_t)c * sa] = cell[l] + dt * td;
+      }
+    }
+  };
+
+  double w[5][ORD];
+  sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { w[l][s] = ld(l, s0 - 1 - hs + s); }); });
+  double prevR[5], prevF[5], prevCell[5], prevSrc = 0.0;
+  if (per || s0 > 0) {                 // right edge of the cell below the segment
+    CellBg<NGLL> bg;
+    bg_of(((s0 - 1) % n + n) % n, bg);
+    double eL[5], sa_;
+    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, dt, dsw, eL, prevR, sa_);
+  }
+  for (int v = s0; v < s1; v++) {
+    sfor<5>([&](auto l) {
+      sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
+      w[l][ORD - 1] = ld(l, v + hs);
+    });
+    CellBg<NGLL> bg;
+    bg_of(v, bg);
+    double eL[5], eR[5], src;
+    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, dt, dsw, eL, eR, src);
+    double F[5];
+    if (!per && v == 0) riemann<DIR>(eL, eL, F);   // wall: both sides from the first cell
+    else riemann<DIR>(prevR, eL, F);
+    face_bg(v, F);
+    if (v > s0) emit(v - 1, prevF, F, prevSrc, prevCell);
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      prevF[l] = F[l];
+      prevR[l] = eR[l];
+      prevCell[l] = w[l][hs];
+    }
+    prevSrc = src;
+  }
+  // top face of the segment
+  double F[5];
+  if (!per && s1 == n) {
+    riemann<DIR>(prevR, prevR, F);                // wall: both sides from the last cell
+  } else {
+    sfor<5>([&](auto l) {
+      sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
+      w[l][ORD - 1] = ld(l, s1 + hs);
+    });
+    CellBg<NGLL> bg;
+    bg_of(s1 % n, bg);
+    double eL[5], eR[5], sa_;
+    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, dt, dsw, eL, eR, sa_);
+    riemann<DIR>(prevR, eL, F);
+  }
+  face_bg(s1, F);
+  emit(s1 - 1, prevF, F, prevSrc, prevCell);
+}
+
+// ---------------------------------------------------------------------------
+// small kernels: tendency application, CFL time step, sums, initial data, fill
+// ---------------------------------------------------------------------------
+__global__ void k_apply(double* __restrict__ st, const double* __restrict__ tend, Grid G, double dt) {
+  int64_t n = (int64_t)5 * G.nz * G.ny * G.nx;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    int i = t % G.nx;
+    int64_t r = t / G.nx;
+    int j = r % G.ny;
+    r /= G.ny;
+    int k = r % G.nz;
+    int l = r / G.nz;
+    st[l * G.sl + (int64_t)(G.hs + k) * G.sz + (int64_t)(G.hs + j) * G.sy + G.hs + i] += dt * tend[t];
+  }
+}
+
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = v;
+}
+
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double block_min(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// S:169-183: per-cell CFL time step, block minima
+__global__ __launch_bounds__(RED_THREADS) void k_dt_partial(const double* __restrict__ st, Grid G, double cfl,
+                                                            double* __restrict__ partial) {
+  __shared__ double sh[RED_THREADS];
+  int64_t n = (int64_t)G.nz * G.ny * G.nx;
+  double m = INFINITY;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    int i = t % G.nx;
+    int64_t rr = t / G.nx;
+    int j = rr % G.ny;
+    int k = rr / G.ny;
+    int64_t c = (int64_t)(G.hs + k) * G.sz + (int64_t)(G.hs + j) * G.sy + G.hs + i;
+    double r = st[0 * G.sl + c] + G.hyDensCells[G.hs + k];
+    double u = st[1 * G.sl + c] / r;
+    double v = st[2 * G.sl + c] / r;
+    double w = st[3 * G.sl + c] / r;
+    double th = (st[4 * G.sl + c] + G.hyDensThetaCells[G.hs + k]) / r;
+    double p = C0 * pow(r * th, GAMMA);
+    double cs = sqrt(GAMMA * p / r);
+    double udt = cfl * G.dx / fmax(fabs(u - cs), fabs(u + cs));
+    double vdt = cfl * G.dy / fmax(fabs(v - cs), fabs(v + cs));
+    double wdt = cfl * G.dz / fmax(fabs(w - cs), fabs(w + cs));
+    m = fmin(m, fmin(fmin(udt, vdt), wdt));
+  }
+  double b = block_min(m, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_min_final(const double* __restrict__ partial, int n,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[RED_THREADS];
+  double m = INFINITY;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmin(m, partial[t]);
+  double b = block_min(m, sh);
+  if (threadIdx.x == 0) out[0] = b;
+}
+
+// per-variable interior sums: block partials [block][5] then a fixed-order final sum
+__global__ __launch_bounds__(RED_THREADS) void k_sum_partial(const double* __restrict__ st, Grid G,
+                                                             double* __restrict__ partial) {
+  __shared__ double sh[5][RED_THREADS];
+  int64_t n = (int64_t)G.nz * G.ny * G.nx;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    int i = t % G.nx;
+    int64_t rr = t / G.nx;
+    int j = rr % G.ny;
+    int k = rr / G.ny;
+    int64_t c = (int64_t)(G.hs + k) * G.sz + (int64_t)(G.hs + j) * G.sy + G.hs + i;
+#pragma unroll
+    for (int l = 0; l < 5; l++) acc[l] += st[l * G.sl + c];
+  }
+#pragma unroll
+  for (int l = 0; l < 5; l++) sh[l][threadIdx.x] = acc[l];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int l = 0; l < 5; l++) sh[l][threadIdx.x] += sh[l][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) partial[blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_sum_final(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  if (threadIdx.x < 5) {
+    double s = 0;
+    for (int b = 0; b < nb; b++) s += partial[b * 5 + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// initial data: Profiles.h:8-73 and the ord^3 GLL quadrature of S:428-485
+// ---------------------------------------------------------------------------
+constexpr double CP = 1004.;
+constexpr double RD = 287.;
+constexpr double P0 = 1.0e5;
+
+__host__ __device__ inline double const_theta_density(double t0, double z) {
+  double exner = 1. - GRAV * z / (CP * t0);
+  double p = pow(exner, CP / RD) * P0;
+  double rt = pow(p / C0, 1. / GAMMA);
+  return rt / t0;
+}
+__host__ __device__ inline double const_theta_pressure(double t0, double z) {
+  double r = const_theta_density(t0, z);
+  return C0 * pow(r * t0, GAMMA);
+}
+__host__ __device__ inline double bvf_theta(double t0, double bvf, double z) { return t0 * exp(bvf * bvf * z / GRAV); }
+__host__ __device__ inline double bvf_density(double t0, double bvf, double z) {
+  double t = bvf_theta(t0, bvf, z);
+  double exner = 1. - GRAV * GRAV / (CP * bvf * bvf) * (t - t0) / (t * t0);
+  double p = pow(exner, CP / RD) * P0;
+  double rt = pow(p / C0, 1. / GAMMA);
+  return rt / t;
+}
+__host__ __device__ inline double bvf_pressure(double t0, double bvf, double z) {
+  double t = bvf_theta(t0, bvf, z);
+  double r = bvf_density(t0, bvf, z);
+  return C0 * pow(r * t, GAMMA);
+}
+
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__host__ __device__ inline double uniform_pm1(uint64_t seed, uint64_t idx) {
+  uint64_t h = splitmix64(splitmix64(seed) ^ idx);
+  return (double)(h >> 11) * 0x1.0p-52 - 1.0;
+}
+
+struct InitDev {
+  int nbub;
+  int shape[4];
+  double x0[4], y0[4], z0[4], xr[4], yr[4], zr[4], amp[4], pwr[4];
+  double noise_amp;
+  uint64_t noise_seed;
+  int nmodes;
+  const double* modes;
+  double u0, v0;
+  int igw;
+  double xlen, ylen, zlen;
+};
+
+template <int ORD>
+__global__ void k_init(double* __restrict__ st, Grid G, InitDev S) {
+  constexpr int hs = (ORD - 1) / 2;
+  int64_t n = (int64_t)G.nz * G.ny * G.nx;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    int i = t % G.nx;
+    int64_t rr = t / G.nx;
+    int j = rr % G.ny;
+    int k = rr / G.ny;
+    int64_t c = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs + i;
+    double sT = 0, sU = 0, sV = 0;
+    double cell_tp = 0;
+    if (S.noise_amp != 0) cell_tp += S.noise_amp * uniform_pm1(S.noise_seed, ((uint64_t)k * G.ny + j) * G.nx + i);
+    if (S.nmodes > 0) {
+      double xc = (i + 0.5) * G.dx, yc = (j + 0.5) * G.dy, zc = (k + 0.5) * G.dz;
+      double acc = 0;
+      for (int m = 0; m < S.nmodes; m++) {
+        const double* md = S.modes + 5 * m;
+        acc += md[3] * cos(2 * M_PI * (md[0] * xc / S.xlen + md[1] * yc / S.ylen + md[2] * zc / S.zlen) + md[4]);
+      }
+      cell_tp += acc;
+    }
+    for (int kk = 0; kk < ORD; kk++) {
+      double zloc = (k + 0.5) * G.dz + Gll<ORD>::pts[kk] * G.dz;
+      double th, rh;
+      if (!S.igw) {
+        th = 300;
+        rh = const_theta_density(th, zloc);
+      } else {
+        rh = bvf_density(300., 0.01, zloc);
+        th = bvf_theta(300., 0.01, zloc);
+      }
+      for (int jj = 0; jj < ORD; jj++)
+        for (int ii = 0; ii < ORD; ii++) {
+          double yloc = G.sim2d ? S.ylen / 2 : (j + 0.5) * G.dy + Gll<ORD>::pts[jj] * G.dy;
+          double xloc = (i + 0.5) * G.dx + Gll<ORD>::pts[ii] * G.dx;
+          double wt = Gll<ORD>::wts[kk] * Gll<ORD>::wts[jj] * Gll<ORD>::wts[ii];
+          double tp = 0;
+          for (int b = 0; b < S.nbub; b++) {
+            double xn = (xloc - S.x0[b]) / S.xr[b], yn = (yloc - S.y0[b]) / S.yr[b], zn = (zloc - S.z0[b]) / S.zr[b];
+            double dist = sqrt(xn * xn + yn * yn + zn * zn);
+            if (S.shape[b] == 0) tp += S.amp[b] * fmax(1. - dist, 0.);
+            else tp += dist <= 1. ? S.amp[b] * pow((cos(M_PI * dist) + 1) / 2, S.pwr[b]) : 0.;
+          }
+          tp += cell_tp;
+          double tt = th + tp;
+          sT += (rh * tt - rh * th) * wt;
+          if (S.u0 != 0) sU += S.u0 * rh * wt;
+          if (S.v0 != 0 && !G.sim2d) sV += S.v0 * rh * wt;
+        }
+    }
+    st[0 * G.sl + c] = 0;
+    st[1 * G.sl + c] = sU;
+    st[2 * G.sl + c] = sV;
+    st[3 * G.sl + c] = 0;
+    st[4 * G.sl + c] = sT;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// compiled (ord, ngll) variants
+// ---------------------------------------------------------------------------
+struct Variant {
+  int ord, ngll;
+  void (*sweep_x[2])(const double*, double*, Grid, double);
+  void (*sweep_y[2])(const double*, double*, Grid, double, int);
+  void (*sweep_z[2])(const double*, double*, Grid, double, int);
+  void (*init)(double*, Grid, InitDev);
+};
+
+#define AWFL_VARIANT(O, N)                                                                        \
+  Variant{O, N,                                                                                   \
+          {k_sweep_x<O, N, false>, k_sweep_x<O, N, true>},                                        \
+          {k_sweep_col<O, N, 1, false>, k_sweep_col<O, N, 1, true>},                              \
+          {k_sweep_col<O, N, 2, false>, k_sweep_col<O, N, 2, true>},                              \
+          k_init<O>}
+
+const Variant kVariants[] = {AWFL_VARIANT(3, 2), AWFL_VARIANT(5, 3), AWFL_VARIANT(9, 5)};
+
+const Variant* find_variant(int ord, int ngll) {
+  for (const Variant& v : kVariants)
+    if (v.ord == ord && v.ngll == ngll) return &v;
+  return nullptr;
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct awfl_ctx {
+  awfl_config cfg{};
+  const Variant* var = nullptr;
+  int hs = 0;
+  bool sim2d = false;
+  double dx = 0, dy = 0, dz = 0;
+  hipStream_t stream = nullptr;
+  int dimSwitch = 1;
+  double dtInit = 0;
+  int device = 0;
+  std::vector<double> hy[8];
+  double* d_hy = nullptr;
+  double* d_buf[2] = {nullptr, nullptr};
+  double* d_red = nullptr;          // reduction partials
+  int n_red = 0;
+  std::string err;
+  Grid grid() const;
+  int64_t state_elems() const {
+    return (int64_t)5 * (cfg.nz + 2 * hs) * (cfg.ny + 2 * hs) * (cfg.nx + 2 * hs);
+  }
+  int64_t tend_elems() const { return (int64_t)5 * cfg.nz * cfg.ny * cfg.nx; }
+};
+
+Grid awfl_ctx::grid() const {
+  Grid G{};
+  G.nx = cfg.nx; G.ny = cfg.ny; G.nz = cfg.nz; G.hs = hs;
+  int64_t nxh = cfg.nx + 2 * hs, nyh = cfg.ny + 2 * hs, nzh = cfg.nz + 2 * hs;
+  G.sy = nxh; G.sz = nxh * nyh; G.sl = nxh * nyh * nzh;
+  G.bc[0] = cfg.bc_x; G.bc[1] = cfg.bc_y; G.bc[2] = cfg.bc_z;
+  G.weno_s = cfg.weno_scalars; G.weno_w = cfg.weno_winds; G.sim2d = sim2d;
+  G.dx = dx; G.dy = dy; G.dz = dz;
+  int nzh_i = (int)nzh, ng = cfg.nz * cfg.ngll;
+  G.hyDensCells = d_hy;
+  G.hyDensThetaCells = d_hy + nzh_i;
+  G.hyDensGLL = d_hy + 2 * nzh_i;
+  G.hyDensThetaGLL = d_hy + 2 * nzh_i + ng;
+  G.hyPressureGLL = d_hy + 2 * nzh_i + 2 * ng;
+  return G;
+}
+
+static int fail(awfl_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                       \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail((ctx), AWFL_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_CHECK(ctx)                                                                        \
+  do {                                                                                           \
+    hipError_t e_ = hipGetLastError();                                                           \
+    if (e_ != hipSuccess) return fail((ctx), AWFL_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// S:370-425: hydrostatic background, FP64 on the host
+static void build_background(awfl_ctx* c) {
+  int hs = c->hs, ord = c->cfg.ord, ngll = c->cfg.ngll, nz = c->cfg.nz, nzh = nz + 2 * hs;
+  bool igw = c->cfg.data_spec == 4;
+  const double* pts_o = nullptr; const double* wts_o = nullptr; const double* pts_g = nullptr;
+  switch (ord) {
+    case 3: pts_o = awfl_gll_pts_3; wts_o = awfl_gll_wts_3; break;
+    case 5: pts_o = awfl_gll_pts_5; wts_o = awfl_gll_wts_5; break;
+    case 7: pts_o = awfl_gll_pts_7; wts_o = awfl_gll_wts_7; break;
+    case 9: pts_o = awfl_gll_pts_9; wts_o = awfl_gll_wts_9; break;
+  }
+  switch (ngll) {
+    case 2: pts_g = awfl_gll_pts_2; break; case 3: pts_g = awfl_gll_pts_3; break;
+    case 4: pts_g = awfl_gll_pts_4; break; case 5: pts_g = awfl_gll_pts_5; break;
+    case 6: pts_g = awfl_gll_pts_6; break; case 7: pts_g = awfl_gll_pts_7; break;
+    case 8: pts_g = awfl_gll_pts_8; break; case 9: pts_g = awfl_gll_pts_9; break;
+  }
+  for (int w = 0; w < 4; w++) c->hy[w].assign(nzh, 0.0);
+  for (int w = 4; w < 8; w++) c->hy[w].assign((size_t)nz * ngll, 0.0);
+  double dz = c->dz;
+  for (int k = 0; k < nzh; k++) {
+    for (int kk = 0; kk < ord; kk++) {
+      double zloc = (k - hs + 0.5) * dz + pts_o[kk] * dz;
+      double th, rh, ph;
+      if (!igw) { th = 300; rh = const_theta_density(th, zloc); ph = const_theta_pressure(th, zloc); }
+      else { th = bvf_theta(300., 0.01, zloc); rh = bvf_density(300., 0.01, zloc); ph = bvf_pressure(300., 0.01, zloc); }
+      double wt = wts_o[kk];
+      c->hy[0][k] += rh * wt;
+      c->hy[2][k] += th * wt;
+      c->hy[3][k] += rh * th * wt;
+      c->hy[1][k] += ph * wt;
+    }
+  }
+  for (int k = 0; k < nz; k++)
+    for (int kk = 0; kk < ngll; kk++) {
+      double zloc = (k + 0.5) * dz + pts_g[kk] * dz;
+      double th, rh, ph;
+      if (!igw) { th = 300; rh = const_theta_density(th, zloc); ph = const_theta_pressure(th, zloc); }
+      else { th = bvf_theta(300., 0.01, zloc); rh = bvf_density(300., 0.01, zloc); ph = bvf_pressure(300., 0.01, zloc); }
+      c->hy[4][(size_t)k * ngll + kk] = rh;
+      c->hy[6][(size_t)k * ngll + kk] = th;
+      c->hy[7][(size_t)k * ngll + kk] = rh * th;
+      c->hy[5][(size_t)k * ngll + kk] = ph;
+    }
+}
+
+extern "C" {
+
+const char* awfl_last_error(const awfl_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int awfl_create(const awfl_config* cfg, awfl_ctx** out) {
+  if (!cfg || !out) return fail(nullptr, AWFL_E_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return fail(nullptr, AWFL_E_ARG, "nx, ny, nz must be >= 1");
+  if (!(cfg->xlen > 0 && cfg->ylen > 0 && cfg->zlen > 0)) return fail(nullptr, AWFL_E_ARG, "lengths must be > 0");
+  for (int b : {cfg->bc_x, cfg->bc_y, cfg->bc_z})
+    if (b != AWFL_BC_PERIODIC && b != AWFL_BC_WALL) return fail(nullptr, AWFL_E_ARG, "bc must be periodic or wall");
+  if (cfg->data_spec < 1 || cfg->data_spec > 4) return fail(nullptr, AWFL_E_ARG, "data_spec must be 1..4");
+  if (cfg->precision != AWFL_FP64) return fail(nullptr, AWFL_E_UNSUPPORTED, "only AWFL_FP64 is available");
+  const Variant* v = find_variant(cfg->ord, cfg->ngll);
+  if (!v) return fail(nullptr, AWFL_E_UNSUPPORTED, "(ord, ngll) not compiled in");
+  int hs = (cfg->ord - 1) / 2;
+  if (cfg->bc_x == AWFL_BC_PERIODIC && cfg->nx < hs) return fail(nullptr, AWFL_E_ARG, "nx < hs");
+  int64_t lds = (int64_t)8 * (5 * (cfg->nx + 2 * hs) + 10 * cfg->nx);
+  if (lds > 160 * 1024) return fail(nullptr, AWFL_E_UNSUPPORTED, "nx too large for the x-line kernel's LDS");
+  awfl_ctx* c = new awfl_ctx();
+  c->cfg = *cfg;
+  c->var = v;
+  c->hs = hs;
+  c->sim2d = cfg->ny == 1;
+  c->dx = cfg->xlen / cfg->nx;
+  c->dy = cfg->ylen / cfg->ny;
+  c->dz = cfg->zlen / cfg->nz;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(nullptr, AWFL_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  }
+  build_background(c);
+  int nzh = cfg->nz + 2 * hs, ng = cfg->nz * cfg->ngll;
+  std::vector<double> packed;
+  packed.insert(packed.end(), c->hy[0].begin(), c->hy[0].end());
+  packed.insert(packed.end(), c->hy[3].begin(), c->hy[3].end());
+  packed.insert(packed.end(), c->hy[4].begin(), c->hy[4].end());
+  packed.insert(packed.end(), c->hy[7].begin(), c->hy[7].end());
+  packed.insert(packed.end(), c->hy[5].begin(), c->hy[5].end());
+  (void)nzh; (void)ng;
+  e = hipMalloc(&c->d_hy, packed.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(c->d_hy, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice);
+  c->n_red = 1024;
+  if (e == hipSuccess) e = hipMalloc(&c->d_red, (size_t)(c->n_red * 5 + 8) * sizeof(double));
+  if (e != hipSuccess) {
+    std::string m = std::string("device allocation: ") + hipGetErrorString(e);
+    awfl_destroy(c);
+    return fail(nullptr, AWFL_E_HIP, m);
+  }
+  *out = c;
+  return AWFL_OK;
+}
+
+void awfl_destroy(awfl_ctx* c) {
+  if (!c) return;
+  if (c->d_hy) (void)hipFree(c->d_hy);
+  if (c->d_red) (void)hipFree(c->d_red);
+  for (double* b : c->d_buf)
+    if (b) (void)hipFree(b);
+  delete c;
+}
+
+int awfl_layout(const awfl_ctx* c, int* hs, int64_t dims_state[4], int64_t dims_tend[4]) {
+  if (!c) return AWFL_E_ARG;
+  if (hs) *hs = c->hs;
+  if (dims_state) {
+    dims_state[0] = 5; dims_state[1] = c->cfg.nz + 2 * c->hs;
+    dims_state[2] = c->cfg.ny + 2 * c->hs; dims_state[3] = c->cfg.nx + 2 * c->hs;
+  }
+  if (dims_tend) {
+    dims_tend[0] = 5; dims_tend[1] = c->cfg.nz; dims_tend[2] = c->cfg.ny; dims_tend[3] = c->cfg.nx;
+  }
+  return AWFL_OK;
+}
+
+int awfl_num_split(const awfl_ctx* c) { return c ? 3 : AWFL_E_ARG; }
+
+int awfl_set_stream(awfl_ctx* c, void* s) {
+  if (!c) return AWFL_E_ARG;
+  c->stream = (hipStream_t)s;
+  return AWFL_OK;
+}
+
+int awfl_get_dim_switch(const awfl_ctx* c) { return c ? c->dimSwitch : AWFL_E_ARG; }
+int awfl_set_dim_switch(awfl_ctx* c, int v) {
+  if (!c) return AWFL_E_ARG;
+  c->dimSwitch = v ? 1 : 0;
+  return AWFL_OK;
+}
+
+int awfl_get_background(const awfl_ctx* c, int which, double* h_out, int64_t cap) {
+  if (!c || !h_out || which < 0 || which > 7) return AWFL_E_ARG;
+  const std::vector<double>& v = c->hy[which];
+  if ((int64_t)v.size() > cap) return AWFL_E_ARG;
+  std::memcpy(h_out, v.data(), v.size() * sizeof(double));
+  return (int)v.size();
+}
+
+int awfl_init_state(awfl_ctx* c, double* d_state, const awfl_init_spec* s) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state || !s) return fail(c, AWFL_E_ARG, "null state or spec");
+  if (s->nbub < 0 || s->nbub > 4) return fail(c, AWFL_E_ARG, "nbub must be 0..4");
+  if (s->nmodes < 0 || (s->nmodes > 0 && !s->modes)) return fail(c, AWFL_E_ARG, "bad modes");
+  InitDev D{};
+  D.nbub = s->nbub;
+  for (int b = 0; b < 4; b++) {
+    D.shape[b] = s->shape[b]; D.x0[b] = s->x0[b]; D.y0[b] = s->y0[b]; D.z0[b] = s->z0[b];
+    D.xr[b] = s->xr[b]; D.yr[b] = s->yr[b]; D.zr[b] = s->zr[b]; D.amp[b] = s->amp[b]; D.pwr[b] = s->pwr[b];
+  }
+  D.noise_amp = s->noise_amp; D.noise_seed = s->noise_seed;
+  D.u0 = s->u0; D.v0 = s->v0; D.igw = c->cfg.data_spec == 4;
+  D.xlen = c->cfg.xlen; D.ylen = c->cfg.ylen; D.zlen = c->cfg.zlen;
+  double* d_modes = nullptr;
+  if (s->nmodes > 0) {
+    HIP_TRY(c, hipMalloc(&d_modes, sizeof(double) * 5 * s->nmodes));
+    HIP_TRY(c, hipMemcpy(d_modes, s->modes, sizeof(double) * 5 * s->nmodes, hipMemcpyHostToDevice));
+  }
+  D.nmodes = s->nmodes; D.modes = d_modes;
+  Grid G = c->grid();
+  int64_t n = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  c->var->init<<<blocks, 256, 0, c->stream>>>(d_state, G, D);
+  hipError_t e = hipGetLastError();
+  if (d_modes) {
+    hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_modes);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e != hipSuccess) return fail(c, AWFL_E_HIP, std::string("init kernel: ") + hipGetErrorString(e));
+  return AWFL_OK;
+}
+
+int awfl_set_time_step(awfl_ctx* c, double dt) {
+  if (!c) return AWFL_E_ARG;
+  c->dtInit = dt;
+  return AWFL_OK;
+}
+
+int awfl_compute_time_step(awfl_ctx* c, const double* d_state, double cfl, double* dt) {
+  if (!c) return AWFL_E_ARG;
+  if (!dt) return fail(c, AWFL_E_ARG, "null dt");
+  if (c->dtInit <= 0) {
+    if (!d_state) return fail(c, AWFL_E_ARG, "null state");
+    Grid G = c->grid();
+    int64_t n = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+    int blocks = (int)std::min<int64_t>((n + RED_THREADS - 1) / RED_THREADS, c->n_red);
+    k_dt_partial<<<blocks, RED_THREADS, 0, c->stream>>>(d_state, G, cfl, c->d_red);
+    LAUNCH_CHECK(c);
+    k_min_final<<<1, RED_THREADS, 0, c->stream>>>(c->d_red, blocks, c->d_red + 5 * c->n_red);
+    LAUNCH_CHECK(c);
+    double h = 0;
+    HIP_TRY(c, hipMemcpyAsync(&h, c->d_red + 5 * c->n_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->dtInit = h;
+  }
+  *dt = c->dtInit;
+  return AWFL_OK;
+}
+
+int awfl_state_sums(awfl_ctx* c, const double* d_state, double* h_out) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state || !h_out) return fail(c, AWFL_E_ARG, "null argument");
+  Grid G = c->grid();
+  int64_t n = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  int blocks = (int)std::min<int64_t>((n + RED_THREADS - 1) / RED_THREADS, c->n_red);
+  k_sum_partial<<<blocks, RED_THREADS, 0, c->stream>>>(d_state, G, c->d_red);
+  LAUNCH_CHECK(c);
+  k_sum_final<<<1, 64, 0, c->stream>>>(c->d_red, blocks, c->d_red + 5 * c->n_red);
+  LAUNCH_CHECK(c);
+  HIP_TRY(c, hipMemcpyAsync(h_out, c->d_red + 5 * c->n_red, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AWFL_OK;
+}
+
+}  // extern "C"
+
+// launch one sweep; tend selects the tendency output (layout [5][nz][ny][nx])
+static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {
+  Grid G = c->grid();
+  const Variant* v = c->var;
+  if (dir == 0) {
+    int nx = c->cfg.nx;
+    int threads = std::min(256, (nx + 63) / 64 * 64);
+    size_t lds = sizeof(double) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx);
+    auto kern = v->sweep_x[tend ? 1 : 0];
+    if (lds > 64 * 1024)
+      HIP_TRY(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    kern<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, out, G, dt);
+    LAUNCH_CHECK(c);
+    return AWFL_OK;
+  }
+  int n = dir == 1 ? c->cfg.ny : c->cfg.nz;
+  int other = dir == 1 ? c->cfg.nz : c->cfg.ny;
+  int bx = (c->cfg.nx + 63) / 64;
+  // enough segments for ~4 waves per SIMD (256 CUs x 4 SIMDs), segments >= 16 cells
+  int64_t layer = (int64_t)bx * other;
+  int nseg = (int)std::max<int64_t>(1, std::min<int64_t>((4096 + layer - 1) / layer, std::max(1, n / 16)));
+  int seglen = (n + nseg - 1) / nseg;
+  nseg = (n + seglen - 1) / seglen;
+  dim3 grid(bx, other, nseg);
+  auto kern = dir == 1 ? v->sweep_y[tend ? 1 : 0] : v->sweep_z[tend ? 1 : 0];
+  kern<<<grid, 64, 0, c->stream>>>(in, out, G, dt, seglen);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
+static int fill(awfl_ctx* c, double* p, int64_t n, double v) {
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  k_fill<<<blocks, 256, 0, c->stream>>>(p, n, v);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
+extern "C" {
+
+int awfl_sweep(awfl_ctx* c, const double* d_in, double* d_out, double dt, int dir) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_in || !d_out) return fail(c, AWFL_E_ARG, "null buffer");
+  if (d_in == d_out) return fail(c, AWFL_E_ARG, "sweeps run out of place: d_in and d_out alias");
+  if (dir < 0 || dir > 2) return fail(c, AWFL_E_ARG, "dir must be 0, 1 or 2");
+  if (dir == 1 && c->sim2d) {
+    HIP_TRY(c, hipMemcpyAsync(d_out, d_in, sizeof(double) * c->state_elems(), hipMemcpyDeviceToDevice, c->stream));
+    return AWFL_OK;
+  }
+  return launch_sweep(c, d_in, d_out, dt, dir, false);
+}
+
+int awfl_compute_tendencies(awfl_ctx* c, const double* d_state, double* d_tend, double dt, int split) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state || !d_tend) return fail(c, AWFL_E_ARG, "null buffer");
+  if (split < 0 || split > 2) return fail(c, AWFL_E_ARG, "split must be 0, 1 or 2");
+  static const int fwd[3] = {0, 1, 2}, bwd[3] = {2, 1, 0};
+  int dir = c->dimSwitch ? fwd[split] : bwd[split];
+  int rc;
+  if (dir == 1 && c->sim2d) rc = fill(c, d_tend, c->tend_elems(), 0.0);   // S:572-575
+  else rc = launch_sweep(c, d_state, d_tend, dt, dir, true);
+  if (rc != AWFL_OK) return rc;
+  if (split == 2) c->dimSwitch = !c->dimSwitch;     // S:593
+  return AWFL_OK;
+}
+
+int awfl_apply_tendencies(awfl_ctx* c, double* d_state, const double* d_tend, double dt, int split) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state || !d_tend) return fail(c, AWFL_E_ARG, "null buffer");
+  if (split < 0 || split > 2) return fail(c, AWFL_E_ARG, "split must be 0, 1 or 2");
+  Grid G = c->grid();
+  int64_t n = c->tend_elems();
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  k_apply<<<blocks, 256, 0, c->stream>>>(d_state, d_tend, G, dt);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
+int awfl_time_step(awfl_ctx* c, double* d_state, double dt) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state) return fail(c, AWFL_E_ARG, "null state");
+  for (int b = 0; b < 2; b++)
+    if (!c->d_buf[b]) HIP_TRY(c, hipMalloc(&c->d_buf[b], sizeof(double) * c->state_elems()));
+  int dirs[3], nd = 0;
+  static const int fwd[3] = {0, 1, 2}, bwd[3] = {2, 1, 0};
+  for (int s = 0; s < 3; s++) {
+    int d = c->dimSwitch ? fwd[s] : bwd[s];
+    if (d == 1 && c->sim2d) continue;                // tendency is zero (S:572-575)
+    dirs[nd++] = d;
+  }
+  const double* cur = d_state;
+  for (int s = 0; s < nd; s++) {
+    double* dst = (s == nd - 1) ? d_state : (cur == c->d_buf[0] ? c->d_buf[1] : c->d_buf[0]);
+    int rc = launch_sweep(c, cur, dst, dt, dirs[s], false);
+    if (rc != AWFL_OK) return rc;
+    cur = dst;
+  }
+  c->dimSwitch = !c->dimSwitch;
+  return AWFL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+"""ctypes binding of the C ABI in include/awfl_dycore.h.
+
+The Python-side mirror of the reference's Spatial operator
+(Spatial_euler3d_cons_expl_cart_fv_Agrid.h) and of Temporal_ader::timeStep,
+operating on torch device tensors as plain device pointers.  Every numerical
+operation runs in the HIP library; if the library is missing the import of a
+``Dycore`` fails loudly -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libawfl_dycore.so")
+
+AWFL_BC_PERIODIC, AWFL_BC_WALL = 0, 1
+AWFL_FP64, AWFL_FP32_PERT = 0, 1
+
+
+class AwflConfig(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xlen", C.c_double), ("ylen", C.c_double), ("zlen", C.c_double),
+                ("bc_x", C.c_int), ("bc_y", C.c_int), ("bc_z", C.c_int),
+                ("weno_scalars", C.c_int), ("weno_winds", C.c_int),
+                ("ord", C.c_int), ("ngll", C.c_int), ("data_spec", C.c_int), ("precision", C.c_int)]
+
+
+class InitSpec(C.Structure):
+    _fields_ = [("nbub", C.c_int), ("shape", C.c_int * 4),
+                ("x0", C.c_double * 4), ("y0", C.c_double * 4), ("z0", C.c_double * 4),
+                ("xr", C.c_double * 4), ("yr", C.c_double * 4), ("zr", C.c_double * 4),
+                ("amp", C.c_double * 4), ("pwr", C.c_double * 4),
+                ("noise_amp", C.c_double), ("noise_seed", C.c_uint64),
+                ("nmodes", C.c_int), ("modes", C.POINTER(C.c_double)),
+                ("u0", C.c_double), ("v0", C.c_double)]
+
+
+def bubble_spec(bubbles=(), noise_amp=0.0, noise_seed=0, modes=None, u0=0.0, v0=0.0):
+    """bubbles: iterable of dicts(shape, x0, y0, z0, xr, yr, zr, amp, pwr)."""
+    s = InitSpec()
+    bubbles = list(bubbles)
+    if len(bubbles) > 4:
+        raise ValueError("at most 4 ellipsoids")
+    s.nbub = len(bubbles)
+    for b, d in enumerate(bubbles):
+        s.shape[b] = d.get("shape", 1)
+        for key in ("x0", "y0", "z0", "xr", "yr", "zr", "amp"):
+            getattr(s, key)[b] = d[key]
+        s.pwr[b] = d.get("pwr", 1.0)
+    s.noise_amp = noise_amp
+    s.noise_seed = noise_seed
+    s._keep = None
+    if modes is not None and len(modes):
+        m = np.ascontiguousarray(np.asarray(modes, dtype=np.float64).reshape(-1, 5))
+        s._keep = m
+        s.nmodes = m.shape[0]
+        s.modes = m.ctypes.data_as(C.POINTER(C.c_double))
+    s.u0, s.v0 = u0, v0
+    return s
+
+
+_lib = None
+
+
+def lib():
+    """Load libawfl_dycore.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("HIP library %s is missing; run `python -m awflcloud_amd.build`" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        P, D, I64 = C.c_void_p, C.c_double, C.c_int64
+        L.awfl_create.argtypes = [C.POINTER(AwflConfig), C.POINTER(P)]
+        L.awfl_destroy.argtypes = [P]
+        L.awfl_last_error.restype = C.c_char_p
+        L.awfl_last_error.argtypes = [P]
+        L.awfl_layout.argtypes = [P, C.POINTER(C.c_int), C.POINTER(I64), C.POINTER(I64)]
+        L.awfl_num_split.argtypes = [P]
+        L.awfl_set_stream.argtypes = [P, P]
+        L.awfl_init_state.argtypes = [P, P, C.POINTER(InitSpec)]
+        L.awfl_compute_time_step.argtypes = [P, P, D, C.POINTER(D)]
+        L.awfl_set_time_step.argtypes = [P, D]
+        L.awfl_compute_tendencies.argtypes = [P, P, P, D, C.c_int]
+        L.awfl_apply_tendencies.argtypes = [P, P, P, D, C.c_int]
+        L.awfl_time_step.argtypes = [P, P, D]
+        L.awfl_sweep.argtypes = [P, P, P, D, C.c_int]
+        L.awfl_get_dim_switch.argtypes = [P]
+        L.awfl_set_dim_switch.argtypes = [P, C.c_int]
+        L.awfl_get_background.argtypes = [P, C.c_int, C.POINTER(D), I64]
+        L.awfl_state_sums.argtypes = [P, P, C.POINTER(D)]
+        _lib = L
+    return _lib
+
+
+class AwflError(RuntimeError):
+    pass
+
+
+class Dycore:
+    """One Spatial/Temporal instance on the current HIP device."""
+
+    def __init__(self, nx, ny, nz, xlen=20000.0, ylen=20000.0, zlen=10000.0, bc=(0, 0, 1), ord=5, ngll=3,
+                 weno_scalars=True, weno_winds=True, data_spec=1, precision=AWFL_FP64, stream=None):
+        import torch
+        self.torch = torch
+        self.L = lib()
+        cfg = AwflConfig(nx, ny, nz, xlen, ylen, zlen, bc[0], bc[1], bc[2], int(weno_scalars), int(weno_winds),
+                         ord, ngll, data_spec, precision)
+        h = C.c_void_p()
+        rc = self.L.awfl_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise AwflError("awfl_create failed (%d): %s" % (rc, self.L.awfl_last_error(None).decode()))
+        self.h = h
+        self.nx, self.ny, self.nz, self.ord, self.ngll = nx, ny, nz, ord, ngll
+        self.hs = (ord - 1) // 2
+        self.dx, self.dy, self.dz = xlen / nx, ylen / ny, zlen / nz
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._check(self.L.awfl_set_stream(self.h, C.c_void_p(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.awfl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise AwflError("awfl call failed (%d): %s" % (rc, self.L.awfl_last_error(self.h).decode()))
+        return rc
+
+    @staticmethod
+    def _ptr(t):
+        return C.c_void_p(t.data_ptr())
+
+    def _chk_state(self, t):
+        if t.dtype != self.torch.float64 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError("state must be a contiguous float64 device tensor")
+        if tuple(t.shape) != self.state_shape:
+            raise ValueError("state shape %s != %s" % (tuple(t.shape), self.state_shape))
+
+    def _chk_tend(self, t):
+        if t.dtype != self.torch.float64 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError("tendency must be a contiguous float64 device tensor")
+        if tuple(t.shape) != self.tend_shape:
+            raise ValueError("tendency shape %s != %s" % (tuple(t.shape), self.tend_shape))
+
+    # --- layout ---
+    @property
+    def state_shape(self):
+        hs = self.hs
+        return (5, self.nz + 2 * hs, self.ny + 2 * hs, self.nx + 2 * hs)
+
+    @property
+    def tend_shape(self):
+        return (5, self.nz, self.ny, self.nx)
+
+    def num_split(self):
+        return self._check(self.L.awfl_num_split(self.h))
+
+    def new_state(self):
+        return self.torch.zeros(self.state_shape, dtype=self.torch.float64, device="cuda")
+
+    def new_tend(self):
+        return self.torch.zeros(self.tend_shape, dtype=self.torch.float64, device="cuda")
+
+    def interior(self, st):
+        hs = self.hs
+        return st[:, hs:hs + self.nz, hs:hs + self.ny, hs:hs + self.nx]
+
+    # --- Spatial ---
+    def init_state(self, spec, state=None):
+        if state is None:
+            state = self.new_state()
+        self._chk_state(state)
+        self._check(self.L.awfl_init_state(self.h, self._ptr(state), C.byref(spec)))
+        return state
+
+    def compute_time_step(self, state, cfl=0.8):
+        self._chk_state(state)
+        dt = C.c_double()
+        self._check(self.L.awfl_compute_time_step(self.h, self._ptr(state), cfl, C.byref(dt)))
+        return dt.value
+
+    def set_time_step(self, dt):
+        self._check(self.L.awfl_set_time_step(self.h, dt))
+
+    def compute_tendencies(self, state, tend, dt, split):
+        self._chk_state(state)
+        self._chk_tend(tend)
+        self._check(self.L.awfl_compute_tendencies(self.h, self._ptr(state), self._ptr(tend), dt, split))
+
+    def apply_tendencies(self, state, tend, dt, split):
+        self._chk_state(state)
+        self._chk_tend(tend)
+        self._check(self.L.awfl_apply_tendencies(self.h, self._ptr(state), self._ptr(tend), dt, split))
+
+    # --- Temporal ---
+    def time_step(self, state, dt):
+        self._chk_state(state)
+        self._check(self.L.awfl_time_step(self.h, self._ptr(state), dt))
+
+    def time_step_unfused(self, state, tend, dt):
+        """Temporal_ader::timeStep through computeTendencies + applyTendencies."""
+        for spl in range(self.num_split()):
+            self.compute_tendencies(state, tend, dt, spl)
+            self.apply_tendencies(state, tend, dt, spl)
+
+    def sweep(self, src, dst, dt, direction):
+        self._chk_state(src)
+        self._chk_state(dst)
+        self._check(self.L.awfl_sweep(self.h, self._ptr(src), self._ptr(dst), dt, direction))
+
+    @property
+    def dim_switch(self):
+        return self._check(self.L.awfl_get_dim_switch(self.h))
+
+    @dim_switch.setter
+    def dim_switch(self, v):
+        self._check(self.L.awfl_set_dim_switch(self.h, int(v)))
+
+    def background(self, which):
+        cap = self.nz * self.ngll + self.nz + 2 * self.hs
+        out = np.zeros(cap)
+        n = self._check(self.L.awfl_get_background(self.h, which, out.ctypes.data_as(C.POINTER(C.c_double)), cap))
+        return out[:n].copy()
+
+    def state_sums(self, state):
+        self._chk_state(state)
+        out = np.zeros(5)
+        self._check(self.L.awfl_state_sums(self.h, self._ptr(state), out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out

@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Throughput of the fused FP64 ADER-WENO step on one MI355X.
+
+Workload: rising thermal bubble, 256 x 256 x 128 cells over 20 x 20 x 10 km,
+periodic x/y, solid walls in z, WENO order 5 with ADER time order 3, cosine
+bubble (2 K, r = 2 km at 10, 10, 2 km) plus seeded +-1e-3 K noise, dt from the
+CFL condition (0.8) computed once and cached.  One step = three split sweeps
+(x,y,z then z,y,x alternately), each one fused kernel.
+
+Prints ONE JSON line.  `value` is cell-updates/s = nx*ny*nz*steps / wall with
+the state resident in HBM.  `roofline` is for the slowest (dominant) sweep
+kernel: algorithmic HBM bytes per launch (read + write 5 FP64 variables per
+cell = 80 B/cell) divided by its mean duration from HIP events recorded on the
+library's stream around every launch inside the timed region.
+`cpu_baseline` is the FP64 CPU oracle (a C restatement of the reference step)
+timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP64_VALU_PEAK_TFS = 78.6    # MI355X FP64 vector
+BYTES_PER_CELL_SWEEP = 80.0  # read + write 5 doubles
+# FP64 flops per cell and sweep of WENO5/ADER3 counted from the reference loop
+# nests (FMA = 2, add/mul/div/sqrt/pow = 1): 779 FMA, 271 div, 5 pow, 1 sqrt.
+FLOPS_PER_CELL_SWEEP = 2412.0
+
+NX, NY, NZ = 256, 256, 128
+
+
+def cpu_baseline(nthreads, target_s=12.0):
+    """Oracle step rate on the 64x64x32 bubble with `nthreads` threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    nx, ny, nz = 64, 64, 32
+    O = pyoracle.Oracle(nx, ny, nz, bc=(0, 0, 1), nthreads=nthreads)
+    spec = pyoracle.make_spec([dict(shape=1, x0=10000.0, y0=10000.0, z0=2000.0, xr=2000.0, yr=2000.0,
+                                    zr=2000.0, amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1)
+    st = O.init_state(spec)
+    dt = O.compute_time_step(st, 0.8)
+    t1 = O.run_steps(st, dt, 1)
+    steps = max(1, min(100, int(target_s / max(t1, 1e-3))))
+    t = O.run_steps(st, dt, steps)
+    return nx * ny * nz * steps / t, steps, t
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from awflcloud_amd import Dycore, bubble_spec
+
+    D = Dycore(NX, NY, NZ, bc=(0, 0, 1), ord=5, ngll=3)
+    spec = bubble_spec([dict(shape=1, x0=10000.0, y0=10000.0, z0=2000.0, xr=2000.0, yr=2000.0, zr=2000.0,
+                             amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1)
+    state = D.init_state(spec)
+    dt = D.compute_time_step(state, 0.8)
+    bufs = [torch.empty_like(state), torch.empty_like(state)]
+    mass0 = D.state_sums(state)[0]
+
+    def step(ds, events=None):
+        """Temporal_ader::timeStep as three fused sweeps ping-ponging through
+        two buffers and ending in `state` (what awfl_time_step does)."""
+        dirs = (0, 1, 2) if ds else (2, 1, 0)
+        cur = state
+        for s, d in enumerate(dirs):
+            dst = state if s == 2 else (bufs[1] if cur is bufs[0] else bufs[0])
+            if events is not None:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                D.sweep(cur, dst, dt, d)
+                e1.record()
+                events[d].append((e0, e1))
+            else:
+                D.sweep(cur, dst, dt, d)
+            cur = dst
+        return not ds
+
+    ds = True
+    for _ in range(args.warmup):
+        ds = step(ds)
+    torch.cuda.synchronize()
+    events = {0: [], 1: [], 2: []}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ds = step(ds, events)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    ncell = NX * NY * NZ
+    value = ncell * args.steps / wall
+    kms = {d: sum(a.elapsed_time(b) for a, b in ev) / len(ev) for d, ev in events.items()}
+    dom = max(kms, key=kms.get)
+    achieved = BYTES_PER_CELL_SWEEP * ncell / (kms[dom] * 1e-3) / 1e9
+    flops = FLOPS_PER_CELL_SWEEP * ncell / (kms[dom] * 1e-3) / 1e12
+    mass1 = D.state_sums(state)[0]
+    finite = bool(torch.isfinite(state).all())
+
+    out = {
+        "metric": "cell-updates/s per ADER step, FP64 vs FP32 at stated error; adjoint:forward time",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "thermal bubble 256x256x128, WENO5/ADER3, FP64, periodic x/y, wall z",
+                   "nx": NX, "ny": NY, "nz": NZ, "ord": 5, "ngll": 3, "dt": dt, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": ["sweep_x", "sweep_y", "sweep_z"][dom],
+                     "kernel_ms": {["x", "y", "z"][d]: kms[d] for d in kms}},
+        "valu": {"achieved_tflops": flops, "peak_tflops": FP64_VALU_PEAK_TFS, "frac": flops / FP64_VALU_PEAK_TFS,
+                 "flops_per_cell_sweep": FLOPS_PER_CELL_SWEEP},
+        "checks": {"finite": finite, "mass_drift_rel": abs(mass1 - mass0) / max(abs(mass0), 1.0)},
+    }
+    if not args.no_cpu:
+        v1, n1, t1 = cpu_baseline(1, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": v1, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                               "sample": "CPU oracle (C restatement, -O3, no FMA), 64x64x32 bubble, "
+                                         "WENO5/ADER3, %d steps in %.1f s, 1 thread" % (n1, t1)}
+        ncores = len(os.sched_getaffinity(0))
+        nthr = min(ncores, 16)
+        vN, nN, tN = cpu_baseline(nthr, args.cpu_seconds / 2)
+        out["cpu_baseline"]["all_cores"] = {"value": vN, "threads": nthr, "host_cpus": ncores,
+                                            "sample": "same workload, %d steps in %.1f s" % (nN, tN)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

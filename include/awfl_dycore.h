@@ -1,0 +1,129 @@
+/* awfl_dycore.h -- C ABI of the MI355X-native ADER-WENO dynamical-core step.
+ *
+ * Drop-in boundary for the Spatial/Temporal concept of the reference
+ * (Spatial_euler3d_cons_expl_cart_fv_Agrid.h, below "S:"; Temporal_ader.h,
+ * below "T:").  Plain C types only: device pointers are `double*` allocated
+ * by the caller (hipMalloc or any HIP allocator), sizes are int64.  Every
+ * entry point returns 0 on success and a negative AWFL_E_* code on failure;
+ * awfl_last_error() gives the message (the reference throws via endrun,
+ * const.h:76-79, a C++ adapter rethrows).
+ *
+ * Threading: one context = one HIP stream; a context is not thread-safe
+ * (like the reference's single-threaded Spatial object).  Calls are
+ * asynchronous on the context's stream except awfl_compute_time_step and
+ * the host-copy helpers, which synchronise (as ParallelMin did, S:182-183).
+ *
+ * Layout (S:34-38, 129-143): state(l,k,j,i) = [5][nz+2hs][ny+2hs][nx+2hs],
+ * x fastest, l = rho', rho*u, rho*v, rho*w, (rho*theta)' (S:69-73); the
+ * tendency is [5][nz][ny][nx].  hs = (ord-1)/2.  The halo cells of the
+ * caller's state are neither read nor written: boundary values (periodic wrap
+ * or solid wall, S:631-662, 994-1025, 1351-1382) are generated on the fly.
+ */
+#ifndef AWFL_DYCORE_H
+#define AWFL_DYCORE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AWFL_OK 0
+#define AWFL_E_ARG -1       /* invalid argument / configuration              */
+#define AWFL_E_HIP -2       /* HIP runtime error                             */
+#define AWFL_E_STATE -3     /* call out of sequence                          */
+#define AWFL_E_UNSUPPORTED -4 /* (ord, ngll) or precision not compiled in    */
+
+#define AWFL_BC_PERIODIC 0  /* S:75 */
+#define AWFL_BC_WALL 1      /* S:76 */
+
+#define AWFL_FP64 0         /* plain FP64 state                              */
+#define AWFL_FP32_PERT 1    /* FP32 perturbation state + Kahan compensation  */
+
+typedef struct awfl_ctx awfl_ctx;
+
+/* Configuration: the YAML keys of Spatial::init (S:197-272) plus the
+ * compile-time ORD/NGLL of const.h:21-27 (nAder = ngll, T:8). */
+typedef struct {
+  int nx, ny, nz;
+  double xlen, ylen, zlen;
+  int bc_x, bc_y, bc_z;
+  int weno_scalars, weno_winds;
+  int ord, ngll;
+  int data_spec;        /* 1 thermal, 2 collision, 3 straka (constant theta0=300 K
+                           background), 4 igw (constant-N background), S:78-81 */
+  int precision;        /* AWFL_FP64 or AWFL_FP32_PERT */
+} awfl_config;
+
+/* Synthetic initial data: sum of ellipsoids (Profiles.h:49-73) integrated with
+ * the ord^3 GLL quadrature of initState (S:428-485), plus per-cell uniform
+ * theta' noise from a counter-based generator and an optional band-limited
+ * theta' field evaluated at cell centres. */
+typedef struct {
+  int nbub;
+  int shape[4];                       /* 0 linear cone, 1 cosine^pwr */
+  double x0[4], y0[4], z0[4], xr[4], yr[4], zr[4], amp[4], pwr[4];
+  double noise_amp;                   /* theta' noise uniform in [-amp, amp) K */
+  uint64_t noise_seed;
+  int nmodes;                         /* modes: host array nmodes x (kx,ky,kz,amp,phase) */
+  const double* modes;
+  double u0, v0;                      /* uniform wind added as rho_h*u0 (igw uses 20) */
+} awfl_init_spec;
+
+/* Spatial::init (S:192-340): builds the transform tables (S:274-315) and the
+ * FP64 hydrostatic background (S:370-425) on the device. */
+int awfl_create(const awfl_config* cfg, awfl_ctx** out);
+/* Spatial::finalize (S:1795) + destructor. */
+void awfl_destroy(awfl_ctx* ctx);
+/* Message of the last failed call on ctx (or of the last failed awfl_create). */
+const char* awfl_last_error(const awfl_ctx* ctx);
+
+/* hs and the dimensions of StateArr / StateTendArr (S:23, 129-143). */
+int awfl_layout(const awfl_ctx* ctx, int* hs, int64_t dims_state[4], int64_t dims_tend[4]);
+/* Spatial::numSplit (S:153-155): always 3. */
+int awfl_num_split(const awfl_ctx* ctx);
+/* Stream the context launches on (hipStream_t as void*; NULL = null stream). */
+int awfl_set_stream(awfl_ctx* ctx, void* hip_stream);
+
+/* Spatial::initState (S:345-500) for the synthetic specs above. */
+int awfl_init_state(awfl_ctx* ctx, double* d_state, const awfl_init_spec* spec);
+
+/* Spatial::computeTimeStep (S:159-187): on the first call (cached dt <= 0)
+ * reduces the CFL time step over the state on the device and caches it; later
+ * calls return the cached value.  Synchronises the context's stream. */
+int awfl_compute_time_step(awfl_ctx* ctx, const double* d_state, double cfl, double* dt);
+/* Reset / override the cached time step (checkpoint restart). */
+int awfl_set_time_step(awfl_ctx* ctx, double dt);
+
+/* Spatial::computeTendencies (S:565-594): tendency of split `split` (0..2),
+ * direction chosen by dimSwitch (x,y,z when set, z,y,x otherwise); toggles
+ * dimSwitch after split 2.  d_state is read only. */
+int awfl_compute_tendencies(awfl_ctx* ctx, const double* d_state, double* d_tend, double dt, int split);
+/* The applySingleTendency contract of Temporal_ader (T:63-65, README.md:19,27,103):
+ * state(l,hs+k,hs+j,hs+i) += dt * tend(l,k,j,i) over the interior. */
+int awfl_apply_tendencies(awfl_ctx* ctx, double* d_state, const double* d_tend, double dt, int split);
+
+/* Temporal_ader::timeStep (T:48-72) fused: three split sweeps, each one
+ * kernel that reconstructs, evolves, fluxes and applies the tendency.  The
+ * result is left in d_state.  Uses two context-owned state-sized buffers. */
+int awfl_time_step(awfl_ctx* ctx, double* d_state, double dt);
+/* One fused split sweep in direction dir (0 x, 1 y, 2 z): d_out = d_in + dt*tend.
+ * d_in and d_out must not alias. */
+int awfl_sweep(awfl_ctx* ctx, const double* d_in, double* d_out, double dt, int dir);
+
+/* dimSwitch of S:93 (checkpoint / restart). */
+int awfl_get_dim_switch(const awfl_ctx* ctx);
+int awfl_set_dim_switch(awfl_ctx* ctx, int v);
+
+/* Hydrostatic background (S:45-52) copied to host: which = 0 hyDensCells,
+ * 1 hyPressureCells, 2 hyThetaCells, 3 hyDensThetaCells (nz+2hs values),
+ * 4 hyDensGLL, 5 hyPressureGLL, 6 hyThetaGLL, 7 hyDensThetaGLL (nz*ngll).
+ * Returns the count written (>= 0) or an error code. */
+int awfl_get_background(const awfl_ctx* ctx, int which, double* h_out, int64_t cap);
+
+/* Global sums over the interior of each state variable, in FP64 on the device
+ * (conservation diagnostics).  h_out[5]. Synchronises. */
+int awfl_state_sums(awfl_ctx* ctx, const double* d_state, double* h_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AWFL_DYCORE_H */
