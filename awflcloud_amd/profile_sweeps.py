@@ -1,0 +1,44 @@
+"""Launch each fused sweep kernel a fixed number of times on the 256x256x128
+thermal bubble, plus one elementwise kernel of known size (read 8N, write 8N
+bytes) that calibrates HBM byte counters.  Meant to run under rocprofv3:
+
+    rocprofv3 --kernel-trace --stats -f csv -d <dir> -- python3 -m awflcloud_amd.profile_sweeps
+    rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -- python3 -m awflcloud_amd.profile_sweeps
+"""
+import argparse
+import sys
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--ord", type=int, default=5)
+    ap.add_argument("--ngll", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--copy-mib", type=int, default=512)
+    a = ap.parse_args()
+    import torch
+    from awflcloud_amd import Dycore, bubble_spec
+    D = Dycore(a.nx, a.ny, a.nz, ord=a.ord, ngll=a.ngll)
+    st = D.init_state(bubble_spec([dict(shape=1, x0=10000.0, y0=10000.0, z0=2000.0, xr=2000.0, yr=2000.0,
+                                        zr=2000.0, amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1))
+    dt = D.compute_time_step(st, 0.8)
+    out = torch.empty_like(st)
+    for d in range(3):
+        for _ in range(a.reps):
+            D.sweep(st, out, dt, d)
+    n = a.copy_mib * 1024 * 1024 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(a.reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print("profile_sweeps: %dx%dx%d ord %d ngll %d, %d launches per sweep; copy probe %d MiB read + %d MiB write"
+          % (a.nx, a.ny, a.nz, a.ord, a.ngll, a.reps, a.copy_mib, a.copy_mib))
+    sys.stdout.flush()
+
+
+if __name__ == "__main__":
+    main()

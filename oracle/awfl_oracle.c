@@ -77,6 +77,28 @@ struct oracle_ctx {
 static double dmax(double a, double b) { return a > b ? a : b; }
 static double dmin(double a, double b) { return a < b ? a : b; }
 
+/* Sensitivity probe (tests only): when g_pow_jitter != 0 every pow() of the
+ * step is moved by one ulp up or down, chosen by a hash of its arguments, to
+ * measure how far ulp-level libm differences alone move the result. */
+static uint64_t g_pow_jitter = 0;
+static uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+static double spow(double x, double y) {
+  double r = pow(x, y);
+  if (g_pow_jitter) {
+    uint64_t b;
+    memcpy(&b, &x, sizeof b);
+    uint64_t h = mix64(b ^ g_pow_jitter);
+    r = nextafter(r, (h & 1) ? INFINITY : -INFINITY);
+  }
+  return r;
+}
+void oracle_set_pow_jitter(uint64_t seed) { g_pow_jitter = seed; }
+
 /* ------------------------------------------------------------------------ */
 /* tables                                                                   */
 /* ------------------------------------------------------------------------ */
@@ -501,7 +523,7 @@ double oracle_compute_time_step(oracle_ctx* c, const double* state, double cfl) 
           double v = ST(idV, hs + k, hs + j, hs + i) / r;
           double w = ST(idW, hs + k, hs + j, hs + i) / r;
           double t = (ST(idT, hs + k, hs + j, hs + i) + c->hyDensThetaCells[hs + k]) / r;
-          double p = C0 * pow(r * t, GAMMA);
+          double p = C0 * spow(r * t, GAMMA);
           double cs = sqrt(GAMMA * p / r);
           double udt = cfl * c->dx / dmax(fabs(u - cs), fabs(u + cs));
           double vdt = cfl * c->dy / dmax(fabs(v - cs), fabs(v + cs));
@@ -775,7 +797,7 @@ static void riemann_x(const double* L, const double* R, double* f) {   /* S:842-
   double v = 0.5 * (q3_L / q1_L + q3_R / q1_R);
   double w = 0.5 * (q4_L / q1_L + q4_R / q1_R);
   double t = 0.5 * (q5_L / q1_L + q5_R / q1_R);
-  double p = C0 * pow(r * t, GAMMA);
+  double p = C0 * spow(r * t, GAMMA);
   double cs2 = GAMMA * p / r;
   double cs = sqrt(cs2);
   double w1, w2, w3;
@@ -789,7 +811,7 @@ static void riemann_x(const double* L, const double* R, double* f) {   /* S:842-
   double q4 = w3 + w * w5 + w * w6;
   double q5 = t * w5 + t * w6;
   f[0] = q2;
-  f[1] = q2 * q2 / q1 + C0 * pow(q5, GAMMA);
+  f[1] = q2 * q2 / q1 + C0 * spow(q5, GAMMA);
   f[2] = q2 * q3 / q1;
   f[3] = q2 * q4 / q1;
   f[4] = q2 * q5 / q1;
@@ -803,7 +825,7 @@ static void riemann_y(const double* L, const double* R, double* f) {   /* S:1202
   double v = 0.5 * (q3_L + q3_R);
   double w = 0.5 * (q4_L + q4_R);
   double t = 0.5 * (q5_L + q5_R);
-  double p = C0 * pow(r * t, GAMMA);
+  double p = C0 * spow(r * t, GAMMA);
   double cs2 = GAMMA * p / r;
   double cs = sqrt(cs2);
   double w1, w2, w3;
@@ -818,7 +840,7 @@ static void riemann_y(const double* L, const double* R, double* f) {   /* S:1202
   double q5 = t * w5 + t * w6;
   f[0] = q3;
   f[1] = q3 * q2 / q1;
-  f[2] = q3 * q3 / q1 + C0 * pow(q5, GAMMA);
+  f[2] = q3 * q3 / q1 + C0 * spow(q5, GAMMA);
   f[3] = q3 * q4 / q1;
   f[4] = q3 * q5 / q1;
 }
@@ -831,7 +853,7 @@ static void riemann_z(const oracle_ctx* c, const double* L, const double* R, int
   double v = 0.5 * (q3_L / q1_L + q3_R / q1_R);
   double w = 0.5 * (q4_L / q1_L + q4_R / q1_R);
   double t = 0.5 * (q5_L / q1_L + q5_R / q1_R);
-  double p = C0 * pow(r * t, GAMMA);
+  double p = C0 * spow(r * t, GAMMA);
   double cs2 = GAMMA * p / r;
   double cs = sqrt(cs2);
   double w1, w2, w3;
@@ -847,7 +869,7 @@ static void riemann_z(const oracle_ctx* c, const double* L, const double* R, int
   f[0] = q4;
   f[1] = q4 * q2 / q1;
   f[2] = q4 * q3 / q1;
-  f[3] = q4 * q4 / q1 + C0 * pow(q5, GAMMA);
+  f[3] = q4 * q4 / q1 + C0 * spow(q5, GAMMA);
   f[4] = q4 * q5 / q1;
   if (k < c->nz) f[3] -= HYGLL(c->hyPressureGLL, k, 0);
   if (k == c->nz) f[3] -= HYGLL(c->hyPressureGLL, c->nz - 1, c->ngll - 1);
@@ -911,7 +933,7 @@ static void tend_x(oracle_ctx* c, double* state, double* tend, double dt) {
           ruv[0][ii] = rr * u * v;
           ruw[0][ii] = rr * u * w;
           rut[0][ii] = rr * u * t;
-          rtg[0][ii] = pow(rr * t, GAMMA);
+          rtg[0][ii] = spow(rr * t, GAMMA);
         }
         if (c->nader > 1) dt_xy(c, 0, r, ru, rv, rw, rt, ruu, ruv, ruw, rut, rtg, dx);
         double r_tavg[N9], ru_tavg[N9];
@@ -1013,7 +1035,7 @@ static void tend_y(oracle_ctx* c, double* state, double* tend, double dt) {
           rvv[0][jj] = rr * v * v;
           rvw[0][jj] = rr * v * w;
           rvt[0][jj] = rr * v * t;
-          rtg[0][jj] = pow(rr * t, GAMMA);
+          rtg[0][jj] = spow(rr * t, GAMMA);
         }
         if (c->nader > 1) dt_xy(c, 1, r, ru, rv, rw, rt, rvu, rvv, rvw, rvt, rtg, dy);
         double r_tavg[N9], rv_tavg[N9];
@@ -1116,7 +1138,7 @@ static void tend_z(oracle_ctx* c, double* state, double* tend, double dt) {
           rwv[0][kk] = rr * w * v;
           rww[0][kk] = rr * w * w;
           rwt[0][kk] = rr * w * t;
-          rtg[0][kk] = pow(rr * t, GAMMA);
+          rtg[0][kk] = spow(rr * t, GAMMA);
           /* S:1439: scalar assignment to the whole SArray (every entry) */
           double sv = -(r[0][kk] - HYGLL(c->hyDensGLL, k, kk)) * GRAV;
           for (int a = 0; a < c->nader; a++)

@@ -96,6 +96,10 @@ typedef struct {
 } oracle_init_spec;
 void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* s, double* state);
 
+/* Sensitivity probe: seed != 0 moves every pow() of the step by one ulp
+ * (direction from a hash of the argument); 0 restores plain pow(). */
+void oracle_set_pow_jitter(uint64_t seed);
+
 /* Counter-based uniform in [-1,1): SplitMix64 of (seed, index). */
 double oracle_uniform_pm1(uint64_t seed, uint64_t idx);
 

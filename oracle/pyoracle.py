@@ -62,6 +62,7 @@ def lib():
         L.oracle_diff_transform.argtypes = [P, C.c_int, D, C.c_double, C.c_int]
         L.oracle_riemann.argtypes = [P, C.c_int, D, D, C.c_int, D]
         L.oracle_init_state.argtypes = [P, C.POINTER(OracleInitSpec), D]
+        L.oracle_set_pow_jitter.argtypes = [C.c_uint64]
         L.oracle_uniform_pm1.restype = C.c_double
         L.oracle_uniform_pm1.argtypes = [C.c_uint64, C.c_uint64]
         _lib = L
@@ -200,6 +201,11 @@ class Oracle:
         f = np.zeros(5)
         self.L.oracle_riemann(self.h, direction, _dp(qL), _dp(qR), kface, _dp(f))
         return f
+
+
+def set_pow_jitter(seed):
+    """Sensitivity probe: move every pow() of the step by one ulp (0 = off)."""
+    lib().oracle_set_pow_jitter(seed)
 
 
 def uniform_pm1(seed, idx):

@@ -28,3 +28,14 @@ def per_var_rel_err(a, b):
         scale = np.abs(b[l]).max()
         out.append(np.abs(a[l] - b[l]).max() / (scale if scale else 1.0))
     return out
+
+
+def full_rel_err(a, b, rbar, rtbar):
+    """Per-variable max-norm error relative to the max norm of the full
+    conserved field: rho = rho_bar + rho', rho*theta = (rho*theta)_bar +
+    (rho*theta)', momenta as stored.  rbar/rtbar broadcast over (k, j, i)."""
+    out = []
+    for l in range(5):
+        ref = b[l] + (rbar if l == 0 else rtbar if l == 4 else 0.0)
+        out.append(np.abs(a[l] - b[l]).max() / max(np.abs(ref).max(), 1e-300))
+    return out

