@@ -4,8 +4,11 @@
 // upwind characteristic Riemann flux.
 //
 // Every routine restates the reference's formula in the reference's operation
-// order (the FP64 CPU oracle follows the same order); the only arithmetic
-// differences are FMA contraction and the device's pow/sqrt.
+// order (the FP64 CPU oracle follows the same order), with three arithmetic
+// changes that keep the result within a few ulp per operation: divisions by
+// constants or by a value used several times become multiplications by a
+// reciprocal (compile-time, launch-time, or rcp() refined to ~1 ulp);
+// (rho theta)^GAMMA uses pow_gamma(); products and sums contract to FMA.
 //   WENO ..................... WenoLimiter.h:12-25, 93-172
 //   reconstruction ........... Spatial_euler3d_cons_expl_cart_fv_Agrid.h (S:) 1800-1833
 //   level-0 products ......... S:704-715, 1067-1078, 1428-1440
@@ -15,6 +18,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "awfl_fastmath.hpp"
 #include "awfl_tables.hpp"
 
 namespace awfl {
@@ -79,31 +83,34 @@ __device__ __forceinline__ void weno_coefs(const double (&u)[ORD], double (&aw)[
   sfor<hs + 1>([&](auto i) {
     sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= W::idl[i] * a[i][ii]; });
   });
-  sfor<ORD>([&](auto ii) { a[hs + 1][ii] /= W::idl[hs + 1]; });
+  constexpr double ridl = 1.0 / W::idl[hs + 1];
+  sfor<ORD>([&](auto ii) { a[hs + 1][ii] *= ridl; });
   // total variation of every candidate (:133-143), sigma blend (:145-151)
   double tv[hs + 2];
   sfor<hs + 1>([&](auto i) { tv[i] = tv_eval<hs + 1>(a[i]); });
   tv[hs + 1] = tv_eval<ORD>(a[hs + 1]);
   double lo_avg = 0.0;
   sfor<hs + 1>([&](auto i) { lo_avg += tv[i]; });
-  lo_avg /= (hs + 1);
+  lo_avg *= 1.0 / (hs + 1);
   tv[hs + 1] = lo_avg + (tv[hs + 1] - lo_avg) * W::sigma;
   // weights ~ idl / TV^2, convexify, map, convexify (:153-161)
   double w[hs + 2];
-  sfor<hs + 2>([&](auto i) { w[i] = W::idl[i] / (tv[i] * tv[i] + WENO_EPS); });
+  sfor<hs + 2>([&](auto i) { w[i] = W::idl[i] * rcp(tv[i] * tv[i] + WENO_EPS); });
   {
     double sum = 0.0;
     sfor<hs + 2>([&](auto i) { sum += w[i]; });
-    sfor<hs + 2>([&](auto i) { w[i] /= (sum + WENO_EPS); });
+    double rs = rcp(sum + WENO_EPS);
+    sfor<hs + 2>([&](auto i) { w[i] *= rs; });
   }
   sfor<hs + 2>([&](auto i) {
     constexpr double d = W::idl[i];
-    w[i] = w[i] * (d + d * d - 3.0 * d * w[i] + w[i] * w[i]) / (d * d + w[i] * (1.0 - 2.0 * d));
+    w[i] = w[i] * (d + d * d - 3.0 * d * w[i] + w[i] * w[i]) * rcp(d * d + w[i] * (1.0 - 2.0 * d));
   });
   {
     double sum = 0.0;
     sfor<hs + 2>([&](auto i) { sum += w[i]; });
-    sfor<hs + 2>([&](auto i) { w[i] /= (sum + WENO_EPS); });
+    double rs = rcp(sum + WENO_EPS);
+    sfor<hs + 2>([&](auto i) { w[i] *= rs; });
   }
   // weighted sum (:163-171); candidates have no coefficients above hs
   sfor<ORD>([&](auto ii) {
@@ -154,6 +161,16 @@ struct CellBg {
   bool wall_bot, wall_top; // z: solid wall and k == 0 / k == nz-1
 };
 
+// Launch constants of one sweep: dt, 1/d along the sweep, the differential
+// transform factors -1/(d (kt+1)) and the time-average weights dt^kt/(kt+1).
+constexpr int KMAX = 9;
+struct SweepK {
+  double dt;
+  double rd;              // 1 / dsw
+  double ndt[KMAX];       // -1 / (dsw (kt+1)), kt = 0..NA-2
+  double tav[KMAX];       // dt^kt / (kt+1), kt = 1..NA-1 (tav[0] unused)
+};
+
 // ---------------------------------------------------------------------------
 // One cell of a DIR sweep (0 x, 1 y, 2 z): stencils of the 5 state variables
 // along the sweep axis -> time-averaged left/right edge states (and, for z,
@@ -161,7 +178,7 @@ struct CellBg {
 // ---------------------------------------------------------------------------
 template <int ORD, int NGLL, int DIR>
 __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s,
-                                           bool weno_w, double dt, double dsw, double (&eL)[5], double (&eR)[5],
+                                           bool weno_w, const SweepK& K, double (&eL)[5], double (&eR)[5],
                                            double& src_avg) {
   constexpr int NA = NGLL;             // nAder = ngll (Temporal_ader.h:8)
   constexpr int NM = 1 + DIR;          // momentum normal to the sweep
@@ -169,6 +186,7 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
   double f[4][NA][NGLL];               // q_nm*var_l/r for l = 1..4
   double g[NA][NGLL];                  // (r t)^gamma, levels >= 1 carried doubled (S:1904-1915)
   double src[NA][NGLL];                // z gravity source
+  double rr[NGLL], rrt[NGLL];          // 1/r and 1/(r t) at level 0
 
   // level 0: reconstruction + background
   sfor<5>([&](auto l) {
@@ -192,13 +210,15 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
   double sv = 0.0;
   sfor<NGLL>([&](auto ii) {
     double r = q[idR][0][ii];
+    rr[ii] = rcp(r);
     double vel[5];
-    vel[1] = q[1][0][ii] / r;
-    vel[2] = q[2][0][ii] / r;
-    vel[3] = q[3][0][ii] / r;
-    vel[4] = q[4][0][ii] / r;
+    sfor<4>([&](auto m) { vel[m + 1] = q[m + 1][0][ii] * rr[ii]; });
     sfor<4>([&](auto m) { f[m][0][ii] = r * vel[NM] * vel[m + 1]; });
-    g[0][ii] = pow(r * vel[4], GAMMA);
+    if constexpr (NA > 1) {
+      // (r t)^gamma: r t = rho*theta up to an ulp
+      g[0][ii] = pow_gamma(q[idT][0][ii]);
+      rrt[ii] = rcp(q[idT][0][ii]);
+    }
     if constexpr (DIR == 2) sv = -(q[idR][0][ii] - bg.rG[ii]) * GRAV;
   });
   if constexpr (DIR == 2) {
@@ -209,7 +229,8 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
 
   // differential transform: levels 1..NA-1
   sfor<NA - 1>([&](auto kt) {
-    constexpr double kt1 = kt + 1.0;
+    constexpr double rkt1 = 1.0 / (kt + 1.0);
+    const double ndt = K.ndt[kt];
     sfor<NGLL>([&](auto ii) {
       double d[5];
       sfor<5>([&](auto l) { d[l] = -0.0; });
@@ -224,7 +245,7 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
               if constexpr (DIR == 2) d[l] += D * (f[m][kt][s] + C0 * g[kt][s] - bg.pG[s]);
               else d[l] += D * (f[m][kt][s] + C0 * g[kt][s]);
             } else {
-              d[l] += D * (f[m][kt][s] + C0 * g[kt][s] / 2);
+              d[l] += D * (f[m][kt][s] + (0.5 * C0) * g[kt][s]);
             }
           } else {
             d[l] += D * f[m][kt][s];
@@ -232,8 +253,8 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
         });
       });
       sfor<5>([&](auto l) {
-        if constexpr (DIR == 2 && l == idW) q[l][kt + 1][ii] = -d[l] / dsw / kt1 + src[kt][ii] / kt1;
-        else q[l][kt + 1][ii] = -d[l] / dsw / kt1;
+        if constexpr (DIR == 2 && l == idW) q[l][kt + 1][ii] = d[l] * ndt + src[kt][ii] * rkt1;
+        else q[l][kt + 1][ii] = d[l] * ndt;
       });
       if constexpr (DIR == 2) {        // S:2051-2055
         if (bg.wall_top) q[idW][kt + 1][NGLL - 1] = 0.0;
@@ -250,17 +271,17 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
             if constexpr (ir == 0) tot += q[NM][ir][ii] * q[l][kt + 1 - ir][ii];   // f(kt+1) is still 0
             else tot += q[NM][ir][ii] * q[l][kt + 1 - ir][ii] - q[idR][ir][ii] * f[m][kt + 1 - ir][ii];
           });
-          f[m][kt + 1][ii] = tot / q[idR][0][ii];
+          f[m][kt + 1][ii] = tot * rr[ii];
         });
         double tg = 0.0;
         sfor<kt + 1>([&](auto ir) {
-          constexpr double wgt = kt1 - ir;
+          constexpr double wgt = kt + 1.0 - ir;
           if constexpr (ir == 0)          // g(kt+1) is still 0
             tg += wgt * (GAMMA * g[ir][ii] * q[idT][kt + 1 - ir][ii]);
           else
             tg += wgt * (GAMMA * g[ir][ii] * q[idT][kt + 1 - ir][ii] - q[idT][ir][ii] * g[kt + 1 - ir][ii]);
         });
-        g[kt + 1][ii] = (GAMMA * g[0][ii] * q[idT][kt + 1][ii] + tg / kt1) / q[idT][0][ii];
+        g[kt + 1][ii] = (GAMMA * g[0][ii] * q[idT][kt + 1][ii] + tg * rkt1) * rrt[ii];
       });
     }
   });
@@ -268,22 +289,18 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
   // time averages of the edge values (S:2142-2177)
   sfor<5>([&](auto l) {
     double a0 = q[l][0][0], a1 = q[l][0][NGLL - 1];
-    double dtmult = dt;
     sfor<NA - 1>([&](auto km) {
       constexpr int kt = km + 1;
-      a0 += q[l][kt][0] * dtmult / (kt + 1.0);
-      a1 += q[l][kt][NGLL - 1] * dtmult / (kt + 1.0);
-      dtmult *= dt;
+      a0 += q[l][kt][0] * K.tav[kt];
+      a1 += q[l][kt][NGLL - 1] * K.tav[kt];
     });
     eL[l] = a0;
     eR[l] = a1;
   });
   if constexpr (DIR == 2) {            // S:1462, 1489-1492
-    double dtmult = dt;
     sfor<NA - 1>([&](auto km) {
       constexpr int kt = km + 1;
-      sfor<NGLL>([&](auto ii) { src[0][ii] += src[kt][ii] * dtmult / (kt + 1.0); });
-      dtmult *= dt;
+      sfor<NGLL>([&](auto ii) { src[0][ii] += src[kt][ii] * K.tav[kt]; });
     });
     double sa = 0.0;
     sfor<NGLL>([&](auto ii) { sa += src[0][ii] * Gll<NGLL>::wts[ii]; });
@@ -312,36 +329,42 @@ __device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[
     ut2 = 0.5 * (L[T2] + R[T2]);
     t = 0.5 * (L[4] + R[4]);
   } else {
-    un = 0.5 * (L[NM] / L[0] + R[NM] / R[0]);
-    ut1 = 0.5 * (L[T1] / L[0] + R[T1] / R[0]);
-    ut2 = 0.5 * (L[T2] / L[0] + R[T2] / R[0]);
-    t = 0.5 * (L[4] / L[0] + R[4] / R[0]);
+    double rL = rcp(L[0]), rR = rcp(R[0]);
+    un = 0.5 * (L[NM] * rL + R[NM] * rR);
+    ut1 = 0.5 * (L[T1] * rL + R[T1] * rR);
+    ut2 = 0.5 * (L[T2] * rL + R[T2] * rR);
+    t = 0.5 * (L[4] * rL + R[4] * rR);
   }
-  double p = C0 * pow(r * t, GAMMA);
-  double cs2 = GAMMA * p / r;
+  double p = C0 * pow_gamma(r * t);
+  double cs2 = GAMMA * p * rcp(r);
   double cs = sqrt(cs2);
+  double rt = rcp(t);
+  double r2c = 0.5 * rcp(cs);
   double w1, w2, w3;
   if (un > 0) {
-    w1 = L[0] - L[4] / t;
-    w2 = L[T1] - ut1 * L[4] / t;
-    w3 = L[T2] - ut2 * L[4] / t;
+    double a = L[4] * rt;
+    w1 = L[0] - a;
+    w2 = L[T1] - ut1 * a;
+    w3 = L[T2] - ut2 * a;
   } else {
-    w1 = R[0] - R[4] / t;
-    w2 = R[T1] - ut1 * R[4] / t;
-    w3 = R[T2] - ut2 * R[4] / t;
+    double a = R[4] * rt;
+    w1 = R[0] - a;
+    w2 = R[T1] - ut1 * a;
+    w3 = R[T2] - ut2 * a;
   }
-  double w5 = un * R[0] / (2 * cs) - R[NM] / (2 * cs) + R[4] / (2 * t);
-  double w6 = -un * L[0] / (2 * cs) + L[NM] / (2 * cs) + L[4] / (2 * t);
+  double w5 = (un * R[0] - R[NM]) * r2c + 0.5 * R[4] * rt;
+  double w6 = (L[NM] - un * L[0]) * r2c + 0.5 * L[4] * rt;
   double q1 = w1 + w5 + w6;
   double qn = un * w1 + (un - cs) * w5 + (un + cs) * w6;
   double qt1 = w2 + ut1 * w5 + ut1 * w6;
   double qt2 = w3 + ut2 * w5 + ut2 * w6;
   double q5 = t * w5 + t * w6;
+  double rq1 = rcp(q1);
   F[0] = qn;
-  F[NM] = qn * qn / q1 + C0 * pow(q5, GAMMA);
-  F[T1] = qn * qt1 / q1;
-  F[T2] = qn * qt2 / q1;
-  F[4] = qn * q5 / q1;
+  F[NM] = qn * qn * rq1 + C0 * pow_gamma(q5);
+  F[T1] = qn * qt1 * rq1;
+  F[T2] = qn * qt2 * rq1;
+  F[4] = qn * q5 * rq1;
 }
 
 }  // namespace awfl

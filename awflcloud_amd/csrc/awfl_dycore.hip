@@ -58,7 +58,7 @@ __device__ __forceinline__ int64_t tend_index(const Grid& G, int l, int k, int j
 // ---------------------------------------------------------------------------
 template <int ORD, int NGLL, bool TEND>
 __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, double* __restrict__ out, Grid G,
-                                                 double dt) {
+                                                 SweepK K) {
   constexpr int hs = (ORD - 1) / 2;
   extern __shared__ double sm[];
   const int nx = G.nx;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
     double st[5][ORD];
     sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { st[l][s] = s_st[l * W + c + s]; }); });
     double eL[5], eR[5], sa;
-    cell_edges<ORD, NGLL, 0>(st, bg, G.weno_s, G.weno_w, dt, G.dx, eL, eR, sa);
+    cell_edges<ORD, NGLL, 0>(st, bg, G.weno_s, G.weno_w, K, eL, eR, sa);
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       s_L[l * nx + c] = eL[l];
@@ -138,10 +138,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
   for (int c = threadIdx.x; c < nx; c += blockDim.x) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
-      double td = -(s_F[l * (nx + 1) + c + 1] - s_F[l * (nx + 1) + c]) / G.dx;
+      double td = -(s_F[l * (nx + 1) + c + 1] - s_F[l * (nx + 1) + c]) * K.rd;
       if (G.sim2d && l == idV) td = 0.0;
       if constexpr (TEND) out[tend_index(G, l, k, j, c)] = td;
-      else out[l * G.sl + row + c] = in[l * G.sl + row + c] + dt * td;
+      else out[l * G.sl + row + c] = in[l * G.sl + row + c] + K.dt * td;
     }
   }
 }
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
 // ---------------------------------------------------------------------------
 template <int ORD, int NGLL, int DIR, bool TEND>
 __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in, double* __restrict__ out, Grid G,
-                                                  double dt, int seglen) {
+                                                  SweepK K, int seglen) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,7 +159,6 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
   const int o = blockIdx.y;           // y sweep: k; z sweep: j
   const int n = DIR == 1 ? G.ny : G.nz;
   const int64_t sa = DIR == 1 ? G.sy : G.sz;
-  const double dsw = DIR == 1 ? G.dy : G.dz;
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   const int64_t base = DIR == 1 ? (int64_t)(hs + o) * G.sz + (int64_t)hs * G.sy + hs + i
                                 : (int64_t)hs * G.sz + (int64_t)(hs + o) * G.sy + hs + i;
@@ -206,14 +205,14 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
   auto emit = [&](int c, const double (&F0)[5], const double (&F1)[5], double src, const double (&cell)[5]) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
-      double td = -(F1[l] - F0[l]) / dsw;
+      double td = -(F1[l] - F0[l]) * K.rd;
       if (DIR == 2 && l == idW) td = src + td;
       if (DIR == 2 && G.sim2d && l == idV) td = 0.0;
       if constexpr (TEND) {
         if constexpr (DIR == 1) out[tend_index(G, l, o, c, i)] = td;
         else out[tend_index(G, l, c, o, i)] = td;
       } else {
-        out[l * G.sl + base + (int64_t)c * sa] = cell[l] + dt * td;
+        out[l * G.sl + base + (int64_t)c * sa] = cell[l] + K.dt * td;
       }
     }
   };
@@ -225,7 +224,7 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
     CellBg<NGLL> bg;
     bg_of(((s0 - 1) % n + n) % n, bg);
     double eL[5], sa_;
-    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, dt, dsw, eL, prevR, sa_);
+    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
   }
   for (int v = s0; v < s1; v++) {
     sfor<5>([&](auto l) {
@@ -235,7 +234,7 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
     CellBg<NGLL> bg;
     bg_of(v, bg);
     double eL[5], eR[5], src;
-    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, dt, dsw, eL, eR, src);
+    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, src);
     double F[5];
     if (!per && v == 0) riemann<DIR>(eL, eL, F);   // wall: both sides from the first cell
     else riemann<DIR>(prevR, eL, F);
@@ -261,7 +260,7 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
     CellBg<NGLL> bg;
     bg_of(s1 % n, bg);
     double eL[5], eR[5], sa_;
-    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, dt, dsw, eL, eR, sa_);
+    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
     riemann<DIR>(prevR, eL, F);
   }
   face_bg(s1, F);
@@ -282,6 +281,11 @@ __global__ void k_apply(double* __restrict__ st, const double* __restrict__ tend
     int l = r / G.nz;
     st[l * G.sl + (int64_t)(G.hs + k) * G.sz + (int64_t)(G.hs + j) * G.sy + G.hs + i] += dt * tend[t];
   }
+}
+
+__global__ void k_eos(const double* __restrict__ rt, double* __restrict__ p, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = C0 * pow_gamma(rt[t]);
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
@@ -492,9 +496,9 @@ __global__ void k_init(double* __restrict__ st, Grid G, InitDev S) {
 // ---------------------------------------------------------------------------
 struct Variant {
   int ord, ngll;
-  void (*sweep_x[2])(const double*, double*, Grid, double);
-  void (*sweep_y[2])(const double*, double*, Grid, double, int);
-  void (*sweep_z[2])(const double*, double*, Grid, double, int);
+  void (*sweep_x[2])(const double*, double*, Grid, SweepK);
+  void (*sweep_y[2])(const double*, double*, Grid, SweepK, int);
+  void (*sweep_z[2])(const double*, double*, Grid, SweepK, int);
   void (*init)(double*, Grid, InitDev);
 };
 
@@ -784,6 +788,16 @@ int awfl_compute_time_step(awfl_ctx* c, const double* d_state, double cfl, doubl
   return AWFL_OK;
 }
 
+int awfl_eos_pressure(awfl_ctx* c, const double* d_rt, double* d_p, int64_t n) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_rt || !d_p || n < 0) return fail(c, AWFL_E_ARG, "bad eos arguments");
+  if (n == 0) return AWFL_OK;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  k_eos<<<blocks, 256, 0, c->stream>>>(d_rt, d_p, n);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
 int awfl_state_sums(awfl_ctx* c, const double* d_state, double* h_out) {
   if (!c) return AWFL_E_ARG;
   if (!d_state || !h_out) return fail(c, AWFL_E_ARG, "null argument");
@@ -801,9 +815,27 @@ int awfl_state_sums(awfl_ctx* c, const double* d_state, double* h_out) {
 
 }  // extern "C"
 
+// launch constants of a sweep along dir with time step dt
+static SweepK sweep_constants(const awfl_ctx* c, double dt, int dir) {
+  SweepK K{};
+  double dsw = dir == 0 ? c->dx : dir == 1 ? c->dy : c->dz;
+  K.dt = dt;
+  K.rd = 1.0 / dsw;
+  double dtmult = dt;
+  for (int kt = 0; kt < KMAX; kt++) {
+    K.ndt[kt] = -1.0 / (dsw * (kt + 1.0));
+    if (kt >= 1) {
+      K.tav[kt] = dtmult / (kt + 1.0);
+      dtmult *= dt;
+    }
+  }
+  return K;
+}
+
 // launch one sweep; tend selects the tendency output (layout [5][nz][ny][nx])
 static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {
   Grid G = c->grid();
+  SweepK K = sweep_constants(c, dt, dir);
   const Variant* v = c->var;
   if (dir == 0) {
     int nx = c->cfg.nx;
@@ -812,7 +844,7 @@ static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, i
     auto kern = v->sweep_x[tend ? 1 : 0];
     if (lds > 64 * 1024)
       HIP_TRY(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    kern<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, out, G, dt);
+    kern<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, out, G, K);
     LAUNCH_CHECK(c);
     return AWFL_OK;
   }
@@ -826,7 +858,7 @@ static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, i
   nseg = (n + seglen - 1) / seglen;
   dim3 grid(bx, other, nseg);
   auto kern = dir == 1 ? v->sweep_y[tend ? 1 : 0] : v->sweep_z[tend ? 1 : 0];
-  kern<<<grid, 64, 0, c->stream>>>(in, out, G, dt, seglen);
+  kern<<<grid, 64, 0, c->stream>>>(in, out, G, K, seglen);
   LAUNCH_CHECK(c);
   return AWFL_OK;
 }

@@ -1,0 +1,122 @@
+// awfl_fastmath.hpp -- FP64 building blocks of the sweep kernels.
+//
+// * rcp(x): 1/x for positive (or negative) normal x from the hardware
+//   reciprocal estimate refined by two Newton-Raphson steps (|err| <~ 1 ulp).
+// * pow_gamma(x): x^GAMMA for positive normal x, GAMMA the FP64 constant
+//   CP/CV of the equation of state p = C0 (rho theta)^GAMMA.  Computed as
+//   exp(GAMMA * log(x)) with log in double-double (atanh series after
+//   reduction of the mantissa to [sqrt(1/2), sqrt(2))), the product with GAMMA
+//   in double-double, and exp by reduction modulo ln2 and a degree-15 Taylor
+//   polynomial.  Max error ~0.85 ulp for x in [1e-3, 1e8] (tested against
+//   200-bit mpmath).  About 70 VALU instructions against ~215 for the general
+//   library pow (which handles negative bases, zeros, infinities, NaNs and
+//   arbitrary exponents that never occur here).
+//
+// The same source compiles for the host, where it is unit-tested.
+#pragma once
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#else
+#define __host__
+#define __device__
+#define __forceinline__ inline
+#endif
+
+#include <cmath>
+
+namespace awfl {
+using std::fma;
+using std::frexp;
+using std::ldexp;
+using std::rint;
+
+
+constexpr double POW_GAMMA = 1.40027894002789400278940027894;
+
+__host__ __device__ __forceinline__ double rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+#else
+  double r = 1.0 / x;
+#endif
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  return r;
+}
+
+// ln 2 split so that k*LN2_HI is exact for |k| < 2^21
+constexpr double LN2_HI = 0x1.62e42fefa3800p-1;
+constexpr double LN2_LO = 0x1.ef35793c76730p-45;
+constexpr double INV_LN2 = 0x1.71547652b82fep+0;
+constexpr double SQRT_HALF = 0x1.6a09e667f3bcdp-1;
+
+// 2/(2k+1), k = 1..11: atanh(s) = s + s^3 (1/3 + s^2/5 + ...), times 2
+constexpr double LOGC[11] = {2.0 / 3, 2.0 / 5, 2.0 / 7, 2.0 / 9, 2.0 / 11, 2.0 / 13,
+                             2.0 / 15, 2.0 / 17, 2.0 / 19, 2.0 / 21, 2.0 / 23};
+// 1/n!, n = 2..15
+constexpr double EXPC[14] = {1.0 / 2,
+                             1.0 / 6,
+                             1.0 / 24,
+                             1.0 / 120,
+                             1.0 / 720,
+                             1.0 / 5040,
+                             1.0 / 40320,
+                             1.0 / 362880,
+                             1.0 / 3628800,
+                             1.0 / 39916800,
+                             1.0 / 479001600,
+                             1.0 / 6227020800.0,
+                             1.0 / 87178291200.0,
+                             1.0 / 1307674368000.0};
+
+__host__ __device__ __forceinline__ double pow_gamma(double x) {
+  // x = m 2^e, m in [sqrt(1/2), sqrt(2))
+  int e;
+  double m = frexp(x, &e);
+  if (m < SQRT_HALF) {
+    m *= 2.0;
+    e -= 1;
+  }
+  // s = (m-1)/(m+1) as sh + sl
+  double num = m - 1.0;                 // exact
+  double uh = m + 1.0;
+  double bb = uh - 1.0;
+  double ul = (1.0 - (uh - bb)) + (m - bb);   // two-sum error of 1 + m
+  double ru = rcp(uh);
+  double sh = num * ru;
+  double rem = fma(-sh, uh, num) - sh * ul;
+  double sl = rem * ru;
+  // log(m) = 2 sh + (2 sl + 2 s^3 P(s^2))
+  double s2 = sh * sh;
+  double P = LOGC[10];
+#pragma unroll
+  for (int i = 9; i >= 0; i--) P = fma(P, s2, LOGC[i]);
+  double tail = sh * s2 * P;
+  double de = (double)e;
+  double t1 = de * LN2_HI;              // exact
+  double lh = 2.0 * sh;
+  double Lh = t1 + lh;
+  double b2 = Lh - t1;
+  double Ll = ((t1 - (Lh - b2)) + (lh - b2)) + (fma(de, LN2_LO, 2.0 * sl) + tail);
+  double L1 = Lh + Ll;
+  double L2 = Ll - (L1 - Lh);
+  // y = GAMMA * L (double-double)
+  double yh = POW_GAMMA * L1;
+  double yl = fma(POW_GAMMA, L1, -yh) + POW_GAMMA * L2;
+  // exp(yh + yl) = 2^k exp(r)
+  double k = rint(yh * INV_LN2);
+  double rh = fma(-k, LN2_HI, yh);      // exact
+  double t = fma(-k, LN2_LO, yl);
+  double r = rh + t;
+  double rl = (rh - r) + t;
+  double Q = EXPC[13];
+#pragma unroll
+  for (int i = 12; i >= 0; i--) Q = fma(Q, r, EXPC[i]);
+  double w = fma(r * r, Q, rl);
+  double er = 1.0 + (r + w);
+  return ldexp(er, (int)k);
+}
+
+}  // namespace awfl

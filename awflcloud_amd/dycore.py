@@ -89,6 +89,7 @@ def lib():
         L.awfl_set_dim_switch.argtypes = [P, C.c_int]
         L.awfl_get_background.argtypes = [P, C.c_int, C.POINTER(D), I64]
         L.awfl_state_sums.argtypes = [P, P, C.POINTER(D)]
+        L.awfl_eos_pressure.argtypes = [P, P, P, I64]
         _lib = L
     return _lib
 
@@ -230,6 +231,14 @@ class Dycore:
         out = np.zeros(cap)
         n = self._check(self.L.awfl_get_background(self.h, which, out.ctypes.data_as(C.POINTER(C.c_double)), cap))
         return out[:n].copy()
+
+    def eos_pressure(self, rt):
+        """p = C0 (rho theta)^GAMMA on the device for a float64 device tensor."""
+        if rt.dtype != self.torch.float64 or not rt.is_cuda or not rt.is_contiguous():
+            raise ValueError("rt must be a contiguous float64 device tensor")
+        p = self.torch.empty_like(rt)
+        self._check(self.L.awfl_eos_pressure(self.h, self._ptr(rt), self._ptr(p), rt.numel()))
+        return p
 
     def state_sums(self, state):
         self._chk_state(state)

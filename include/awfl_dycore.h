@@ -119,6 +119,10 @@ int awfl_set_dim_switch(awfl_ctx* ctx, int v);
  * Returns the count written (>= 0) or an error code. */
 int awfl_get_background(const awfl_ctx* ctx, int which, double* h_out, int64_t cap);
 
+/* Equation of state p = C0 (rho*theta)^GAMMA (S:855, 1776) element-wise on
+ * the device, with the kernels' own power routine: d_p[i] for d_rt[i] > 0. */
+int awfl_eos_pressure(awfl_ctx* ctx, const double* d_rt, double* d_p, int64_t n);
+
 /* Global sums over the interior of each state variable, in FP64 on the device
  * (conservation diagnostics).  h_out[5]. Synchronises. */
 int awfl_state_sums(awfl_ctx* ctx, const double* d_state, double* h_out);

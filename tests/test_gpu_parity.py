@@ -132,10 +132,10 @@ def test_sim2d():
     host, dt = _generic_state(D, O, v0=0.0, nsteps=2)
     st = torch.from_numpy(host).cuda()
     rbar, rtbar = _bg(O)
-    for _ in range(3):
+    for n in range(3):
         D.time_step(st, dt)
         O.time_step(host, dt)
-        assert max(full_rel_err(D.interior(st).cpu().numpy(), O.interior(host), rbar, rtbar)) <= 1e-12
+        assert max(full_rel_err(D.interior(st).cpu().numpy(), O.interior(host), rbar, rtbar)) <= 1e-12 * (n + 1)
     assert rel_err(D.interior(st).cpu().numpy(), O.interior(host)) <= 1e-10
 
 
@@ -157,7 +157,8 @@ def test_time_step_fused_and_unfused(bc):
         ea = max(full_rel_err(D.interior(a).cpu().numpy(), O.interior(host), rbar, rtbar))
         eb = max(full_rel_err(D.interior(b).cpu().numpy(), O.interior(host), rbar, rtbar))
         print("step", step, "fused", ea, "unfused", eb)
-        assert ea <= 1e-12 and eb <= 1e-12
+        # <= 1e-12 per step, accumulating
+        assert ea <= 1e-12 * (step + 1) and eb <= 1e-12 * (step + 1)
     # the two boundaries run the same kernels: they agree far below the oracle gap
     assert rel_err(D.interior(a).cpu().numpy(), D.interior(b).cpu().numpy()) <= 1e-13
 
