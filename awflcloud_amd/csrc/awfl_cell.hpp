@@ -51,6 +51,29 @@ __device__ __forceinline__ double tv_eval(const double (&a)[M]) {
   return r;
 }
 
+// ex[i] = prod_{j != i} b[j] and P = prod_j b[j] (prefix/suffix products)
+template <int N>
+__device__ __forceinline__ void excl_products(const double (&b)[N], double (&ex)[N], double& P) {
+  double pre[N], suf[N];
+  pre[0] = 1.0;
+  sfor<N - 1>([&](auto i) {
+    if constexpr (i == 0) pre[1] = b[0];
+    else pre[i + 1] = pre[i] * b[i];
+  });
+  suf[N - 1] = 1.0;
+  sfor<N - 1>([&](auto m) {
+    constexpr int i = N - 2 - m;
+    if constexpr (i == N - 2) suf[i] = b[N - 1];
+    else suf[i] = suf[i + 1] * b[i + 1];
+  });
+  sfor<N>([&](auto i) {
+    if constexpr (i == 0) ex[i] = suf[0];
+    else if constexpr (i == N - 1) ex[i] = pre[N - 1];
+    else ex[i] = pre[i] * suf[i];
+  });
+  P = pre[N - 1] * b[N - 1];
+}
+
 // ---------------------------------------------------------------------------
 // weno::compute_weno_coefs: ord cell averages u -> ord polynomial coefficients
 // ---------------------------------------------------------------------------
@@ -93,23 +116,37 @@ __device__ __forceinline__ void weno_coefs(const double (&u)[ORD], double (&aw)[
   sfor<hs + 1>([&](auto i) { lo_avg += tv[i]; });
   lo_avg *= 1.0 / (hs + 1);
   tv[hs + 1] = lo_avg + (tv[hs + 1] - lo_avg) * W::sigma;
-  // weights ~ idl / TV^2, convexify, map, convexify (:153-161)
+  // weights ~ idl / TV^2, convexify, map, convexify (:153-161).  Each
+  // normalisation w_i = (a_i / b_i) / (sum_k a_k / b_k + eps) is formed over
+  // the common denominator: a_i prod_{j!=i} b_j / (sum_k a_k prod_{j!=k} b_j
+  // + eps prod_j b_j), one reciprocal instead of hs+3 divisions.
   double w[hs + 2];
-  sfor<hs + 2>([&](auto i) { w[i] = W::idl[i] * rcp(tv[i] * tv[i] + WENO_EPS); });
   {
-    double sum = 0.0;
-    sfor<hs + 2>([&](auto i) { sum += w[i]; });
-    double rs = rcp(sum + WENO_EPS);
+    double D[hs + 2], ex[hs + 2], P;
+    sfor<hs + 2>([&](auto i) { D[i] = tv[i] * tv[i] + WENO_EPS; });
+    excl_products<hs + 2>(D, ex, P);
+    double N = 0.0;
+    sfor<hs + 2>([&](auto i) {
+      w[i] = W::idl[i] * ex[i];
+      N += w[i];
+    });
+    double rs = rcp(fma(WENO_EPS, P, N));
     sfor<hs + 2>([&](auto i) { w[i] *= rs; });
   }
-  sfor<hs + 2>([&](auto i) {
-    constexpr double d = W::idl[i];
-    w[i] = w[i] * (d + d * d - 3.0 * d * w[i] + w[i] * w[i]) * rcp(d * d + w[i] * (1.0 - 2.0 * d));
-  });
   {
-    double sum = 0.0;
-    sfor<hs + 2>([&](auto i) { sum += w[i]; });
-    double rs = rcp(sum + WENO_EPS);
+    double A[hs + 2], E[hs + 2], ex[hs + 2], P;
+    sfor<hs + 2>([&](auto i) {
+      constexpr double d = W::idl[i];
+      A[i] = w[i] * (d + d * d - 3.0 * d * w[i] + w[i] * w[i]);
+      E[i] = d * d + w[i] * (1.0 - 2.0 * d);
+    });
+    excl_products<hs + 2>(E, ex, P);
+    double M = 0.0;
+    sfor<hs + 2>([&](auto i) {
+      w[i] = A[i] * ex[i];
+      M += w[i];
+    });
+    double rs = rcp(fma(WENO_EPS, P, M));
     sfor<hs + 2>([&](auto i) { w[i] *= rs; });
   }
   // weighted sum (:163-171); candidates have no coefficients above hs
@@ -152,13 +189,27 @@ __device__ __forceinline__ void reconstruct(const double (&u)[ORD], bool doweno,
 // Background seen by one cell of a sweep.  X and Y add the cell averages
 // hyDensCells/hyDensThetaCells (S:680, 697, 1043, 1060); Z adds the GLL point
 // values and subtracts the background pressure (S:1400, 1421, 2042).
+// Each background rho*theta value B also comes with B^GAMMA as a double-double
+// (h, l) and 1/B, the reference point of pow_gamma_near.
+struct PowRef {
+  double h, l, rb, b;
+};
+
 template <int NGLL>
 struct CellBg {
   double r, rt;            // x/y: cell averages of the background
+  PowRef pt;               // x/y: power reference of rt
   double rG[NGLL];         // z: hyDensGLL(k, :)
   double rtG[NGLL];        // z: hyDensThetaGLL(k, :)
   double pG[NGLL];         // z: hyPressureGLL(k, :)
+  PowRef ptG[NGLL];        // z: power references of rtG
   bool wall_bot, wall_top; // z: solid wall and k == 0 / k == nz-1
+};
+
+// power references of a face: for (r t) of the averaged state and for the
+// recomposed q5 of the Riemann flux
+struct FaceRef {
+  PowRef rt, q5;
 };
 
 // Launch constants of one sweep: dt, 1/d along the sweep, the differential
@@ -189,6 +240,7 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
   double rr[NGLL], rrt[NGLL];          // 1/r and 1/(r t) at level 0
 
   // level 0: reconstruction + background
+  double et[NGLL];                     // (rho theta)'/B at the GLL points
   sfor<5>([&](auto l) {
     double gv[NGLL];
     reconstruct<ORD, NGLL>(st[l], (l == idR || l == idT) ? weno_s : weno_w, gv);
@@ -196,9 +248,11 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
   });
   sfor<NGLL>([&](auto ii) {
     if constexpr (DIR == 2) {
+      et[ii] = q[idT][0][ii] * bg.ptG[ii].rb;
       q[idR][0][ii] += bg.rG[ii];
       q[idT][0][ii] += bg.rtG[ii];
     } else {
+      et[ii] = q[idT][0][ii] * bg.pt.rb;
       q[idR][0][ii] += bg.r;
       q[idT][0][ii] += bg.rt;
     }
@@ -216,7 +270,8 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
     sfor<4>([&](auto m) { f[m][0][ii] = r * vel[NM] * vel[m + 1]; });
     if constexpr (NA > 1) {
       // (r t)^gamma: r t = rho*theta up to an ulp
-      g[0][ii] = pow_gamma(q[idT][0][ii]);
+      const PowRef& pr = DIR == 2 ? bg.ptG[ii] : bg.pt;
+      g[0][ii] = pow_gamma_near(q[idT][0][ii], et[ii], pr.h, pr.l);
       rrt[ii] = rcp(q[idT][0][ii]);
     }
     if constexpr (DIR == 2) sv = -(q[idR][0][ii] - bg.rG[ii]) * GRAV;
@@ -316,7 +371,8 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
 // normal velocity test is strict (u > 0).
 // ---------------------------------------------------------------------------
 template <int DIR>
-__device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], double (&F)[5]) {
+__device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], const FaceRef& fr,
+                                        double (&F)[5]) {
   constexpr int NM = 1 + DIR;
   // tangential momentum indices in the order the reference uses them
   constexpr int T1 = DIR == 0 ? 2 : 1;
@@ -335,7 +391,7 @@ __device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[
     ut2 = 0.5 * (L[T2] * rL + R[T2] * rR);
     t = 0.5 * (L[4] * rL + R[4] * rR);
   }
-  double p = C0 * pow_gamma(r * t);
+  double p = C0 * pow_gamma_near(r * t, fma(r, t, -fr.rt.b) * fr.rt.rb, fr.rt.h, fr.rt.l);
   double cs2 = GAMMA * p * rcp(r);
   double cs = sqrt(cs2);
   double rt = rcp(t);
@@ -361,7 +417,7 @@ __device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[
   double q5 = t * w5 + t * w6;
   double rq1 = rcp(q1);
   F[0] = qn;
-  F[NM] = qn * qn * rq1 + C0 * pow_gamma(q5);
+  F[NM] = qn * qn * rq1 + C0 * pow_gamma_near(q5, (q5 - fr.q5.b) * fr.q5.rb, fr.q5.h, fr.q5.l);
   F[T1] = qn * qt1 * rq1;
   F[T2] = qn * qt2 * rq1;
   F[4] = qn * q5 * rq1;

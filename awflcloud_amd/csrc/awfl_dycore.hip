@@ -47,6 +47,10 @@ struct Grid {
   const double* hyDensGLL;       // nz*ngll
   const double* hyDensThetaGLL;
   const double* hyPressureGLL;
+  const PowRef* prT;             // [nz]   x/y: B = hyDensThetaCells(hs+k)
+  const PowRef* prRT;            // [nz]   y faces: B = hyDensCells * hyDensThetaCells
+  const PowRef* prG;             // [nz*ngll] z: B = hyDensThetaGLL(k, kk)
+  const PowRef* prF;             // [nz+1] z faces: B at the face
 };
 
 __device__ __forceinline__ int64_t tend_index(const Grid& G, int l, int k, int j, int i) {
@@ -56,8 +60,15 @@ __device__ __forceinline__ int64_t tend_index(const Grid& G, int l, int k, int j
 // ---------------------------------------------------------------------------
 // x sweep: one workgroup per x-line
 // ---------------------------------------------------------------------------
+#ifndef AWFL_X_WAVES
+#define AWFL_X_WAVES 1
+#endif
+#ifndef AWFL_COL_WAVES
+#define AWFL_COL_WAVES 1
+#endif
+
 template <int ORD, int NGLL, bool TEND>
-__global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, double* __restrict__ out, Grid G,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVES))) void k_sweep_x(const double* __restrict__ in, double* __restrict__ out, Grid G,
                                                  SweepK K) {
   constexpr int hs = (ORD - 1) / 2;
   extern __shared__ double sm[];
@@ -96,7 +107,11 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
   CellBg<NGLL> bg;
   bg.r = G.hyDensCells[hs + k];
   bg.rt = G.hyDensThetaCells[hs + k];
+  bg.pt = G.prT[k];
   bg.wall_bot = bg.wall_top = false;
+  FaceRef fr;
+  fr.rt = bg.pt;
+  fr.q5 = bg.pt;
   for (int c = threadIdx.x; c < nx; c += blockDim.x) {
     double st[5][ORD];
     sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { st[l][s] = s_st[l * W + c + s]; }); });
@@ -110,7 +125,9 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
   }
   __syncthreads();
 
-  for (int f = threadIdx.x; f <= nx; f += blockDim.x) {
+  // periodic: face nx has the inputs of face 0 (S:819-820), so it is not recomputed
+  const int nfaces = per ? nx : nx + 1;
+  for (int f = threadIdx.x; f < nfaces; f += blockDim.x) {
     int a, b;                 // qL from the right edge of cell a, qR from the left edge of cell b
     bool aL = false, bR = false;
     if (per) {
@@ -129,7 +146,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
       qL[l] = aL ? s_L[l * nx + a] : s_R[l * nx + a];
       qR[l] = bR ? s_R[l * nx + b] : s_L[l * nx + b];
     }
-    riemann<0>(qL, qR, F);
+    riemann<0>(qL, qR, fr, F);
 #pragma unroll
     for (int l = 0; l < 5; l++) s_F[l * (nx + 1) + f] = F[l];
   }
@@ -138,7 +155,8 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
   for (int c = threadIdx.x; c < nx; c += blockDim.x) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
-      double td = -(s_F[l * (nx + 1) + c + 1] - s_F[l * (nx + 1) + c]) * K.rd;
+      const int cp = (per && c + 1 == nx) ? 0 : c + 1;
+      double td = -(s_F[l * (nx + 1) + cp] - s_F[l * (nx + 1) + c]) * K.rd;
       if (G.sim2d && l == idV) td = 0.0;
       if constexpr (TEND) out[tend_index(G, l, k, j, c)] = td;
       else out[l * G.sl + row + c] = in[l * G.sl + row + c] + K.dt * td;
@@ -150,7 +168,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const double* __restrict__ in, 
 // y / z sweeps: one lane per x-column walking a segment of its column
 // ---------------------------------------------------------------------------
 template <int ORD, int NGLL, int DIR, bool TEND>
-__global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in, double* __restrict__ out, Grid G,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAVES))) void k_sweep_col(const double* __restrict__ in, double* __restrict__ out, Grid G,
                                                   SweepK K, int seglen) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
@@ -185,16 +203,29 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
     if constexpr (DIR == 1) {
       bg.r = G.hyDensCells[hs + o];
       bg.rt = G.hyDensThetaCells[hs + o];
+      bg.pt = G.prT[o];
       bg.wall_bot = bg.wall_top = false;
     } else {
       sfor<NGLL>([&](auto ii) {
         bg.rG[ii] = G.hyDensGLL[kc * NGLL + ii];
         bg.rtG[ii] = G.hyDensThetaGLL[kc * NGLL + ii];
         bg.pG[ii] = G.hyPressureGLL[kc * NGLL + ii];
+        bg.ptG[ii] = G.prG[kc * NGLL + ii];
       });
       bg.wall_bot = !per && kc == 0;
       bg.wall_top = !per && kc == n - 1;
     }
+  };
+  auto face_ref = [&](int f) {
+    FaceRef fr;
+    if constexpr (DIR == 1) {
+      fr.rt = G.prRT[o];
+      fr.q5 = G.prT[o];
+    } else {
+      fr.rt = G.prF[f];
+      fr.q5 = fr.rt;
+    }
+    return fr;
   };
   auto face_bg = [&](int f, double (&F)[5]) {   // S:1628-1629
     if constexpr (DIR == 2) {
@@ -236,8 +267,9 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
     double eL[5], eR[5], src;
     cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, src);
     double F[5];
-    if (!per && v == 0) riemann<DIR>(eL, eL, F);   // wall: both sides from the first cell
-    else riemann<DIR>(prevR, eL, F);
+    FaceRef fr = face_ref(v);
+    if (!per && v == 0) riemann<DIR>(eL, eL, fr, F);   // wall: both sides from the first cell
+    else riemann<DIR>(prevR, eL, fr, F);
     face_bg(v, F);
     if (v > s0) emit(v - 1, prevF, F, prevSrc, prevCell);
 #pragma unroll
@@ -250,8 +282,9 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
   }
   // top face of the segment
   double F[5];
+  FaceRef fr = face_ref(s1);
   if (!per && s1 == n) {
-    riemann<DIR>(prevR, prevR, F);                // wall: both sides from the last cell
+    riemann<DIR>(prevR, prevR, fr, F);            // wall: both sides from the last cell
   } else {
     sfor<5>([&](auto l) {
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
@@ -261,7 +294,7 @@ __global__ __launch_bounds__(64) void k_sweep_col(const double* __restrict__ in,
     bg_of(s1 % n, bg);
     double eL[5], eR[5], sa_;
     cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
-    riemann<DIR>(prevR, eL, F);
+    riemann<DIR>(prevR, eL, fr, F);
   }
   face_bg(s1, F);
   emit(s1 - 1, prevF, F, prevSrc, prevCell);
@@ -536,6 +569,7 @@ struct awfl_ctx {
   int device = 0;
   std::vector<double> hy[8];
   double* d_hy = nullptr;
+  PowRef* d_pref = nullptr;         // power references: prT | prRT | prG | prF
   double* d_buf[2] = {nullptr, nullptr};
   double* d_red = nullptr;          // reduction partials
   int n_red = 0;
@@ -561,6 +595,10 @@ Grid awfl_ctx::grid() const {
   G.hyDensGLL = d_hy + 2 * nzh_i;
   G.hyDensThetaGLL = d_hy + 2 * nzh_i + ng;
   G.hyPressureGLL = d_hy + 2 * nzh_i + 2 * ng;
+  G.prT = d_pref;
+  G.prRT = d_pref + cfg.nz;
+  G.prG = d_pref + 2 * cfg.nz;
+  G.prF = d_pref + 2 * cfg.nz + ng;
   return G;
 }
 
@@ -628,6 +666,30 @@ static void build_background(awfl_ctx* c) {
     }
 }
 
+// B^GAMMA as a double-double (long double powl, 64-bit mantissa) and 1/B
+static PowRef power_ref(double B) {
+  long double v = powl((long double)B, (long double)GAMMA);
+  PowRef r;
+  r.h = (double)v;
+  r.l = (double)(v - (long double)r.h);
+  r.rb = 1.0 / B;
+  r.b = B;
+  return r;
+}
+
+// reference points of pow_gamma_near, laid out prT[nz] | prRT[nz] | prG[nz*ngll] | prF[nz+1]
+static std::vector<PowRef> power_references(const awfl_ctx* c) {
+  int nz = c->cfg.nz, ngll = c->cfg.ngll, hs = c->hs;
+  std::vector<PowRef> out;
+  for (int k = 0; k < nz; k++) out.push_back(power_ref(c->hy[3][hs + k]));
+  for (int k = 0; k < nz; k++) out.push_back(power_ref(c->hy[0][hs + k] * c->hy[3][hs + k]));
+  for (int k = 0; k < nz; k++)
+    for (int kk = 0; kk < ngll; kk++) out.push_back(power_ref(c->hy[7][(size_t)k * ngll + kk]));
+  for (int f = 0; f <= nz; f++)
+    out.push_back(power_ref(f < nz ? c->hy[7][(size_t)f * ngll] : c->hy[7][(size_t)(nz - 1) * ngll + ngll - 1]));
+  return out;
+}
+
 extern "C" {
 
 const char* awfl_last_error(const awfl_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
@@ -671,6 +733,9 @@ int awfl_create(const awfl_config* cfg, awfl_ctx** out) {
   (void)nzh; (void)ng;
   e = hipMalloc(&c->d_hy, packed.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(c->d_hy, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice);
+  std::vector<PowRef> pref = power_references(c);
+  if (e == hipSuccess) e = hipMalloc(&c->d_pref, pref.size() * sizeof(PowRef));
+  if (e == hipSuccess) e = hipMemcpy(c->d_pref, pref.data(), pref.size() * sizeof(PowRef), hipMemcpyHostToDevice);
   c->n_red = 1024;
   if (e == hipSuccess) e = hipMalloc(&c->d_red, (size_t)(c->n_red * 5 + 8) * sizeof(double));
   if (e != hipSuccess) {
@@ -685,6 +750,7 @@ int awfl_create(const awfl_config* cfg, awfl_ctx** out) {
 void awfl_destroy(awfl_ctx* c) {
   if (!c) return;
   if (c->d_hy) (void)hipFree(c->d_hy);
+  if (c->d_pref) (void)hipFree(c->d_pref);
   if (c->d_red) (void)hipFree(c->d_red);
   for (double* b : c->d_buf)
     if (b) (void)hipFree(b);

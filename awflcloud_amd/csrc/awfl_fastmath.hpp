@@ -29,6 +29,7 @@ using std::fma;
 using std::frexp;
 using std::ldexp;
 using std::rint;
+using std::fabs;
 
 
 constexpr double POW_GAMMA = 1.40027894002789400278940027894;
@@ -117,6 +118,39 @@ __host__ __device__ __forceinline__ double pow_gamma(double x) {
   double w = fma(r * r, Q, rl);
   double er = 1.0 + (r + w);
   return ldexp(er, (int)k);
+}
+
+// (1+e)^GAMMA by its binomial series, degree 12: truncation < 3e-19 relative
+// for |e| <= 1/16.  Coefficients c_n = binom(GAMMA, n), c_0 = 1.
+struct BinomGamma {
+  double c[13];
+  constexpr BinomGamma() : c() {
+    c[0] = 1.0;
+    for (int n = 1; n <= 12; n++) c[n] = c[n - 1] * (POW_GAMMA - (n - 1)) / n;
+  }
+};
+constexpr BinomGamma BINOM_GAMMA{};
+constexpr double NEAR_MAX = 0.0625;
+
+// x^GAMMA for x = B (1 + e) near a reference value B whose power is known as
+// a double-double Bh + Bl = B^GAMMA: B^GAMMA (1 + s) with s = (1+e)^GAMMA - 1
+// from the series, added as Bh + (Bh s + Bl (1+s)) so only the last addition
+// rounds at full size (~0.6 ulp).  Outside |e| <= 1/16 the general
+// pow_gamma(x) is used.
+// out-of-line general path, so the five inlined uses of the fast path do not
+// each carry its code and constants
+__host__ __device__ __attribute__((noinline)) inline double pow_gamma_far(double x) { return pow_gamma(x); }
+
+__host__ __device__ __forceinline__ double pow_gamma_near(double x, double e, double Bh, double Bl) {
+  if (fabs(e) <= NEAR_MAX) {
+    double q = BINOM_GAMMA.c[12];
+#pragma unroll
+    for (int n = 11; n >= 1; n--) q = fma(q, e, BINOM_GAMMA.c[n]);
+    double sv = q * e;
+    double t = fma(Bl, sv, Bl);
+    return Bh + fma(Bh, sv, t);
+  }
+  return pow_gamma_far(x);
 }
 
 }  // namespace awfl

@@ -6,3 +6,7 @@ extern "C" void probe_pow_gamma(const double* x, double* y, long n) {
 extern "C" void probe_rcp(const double* x, double* y, long n) {
   for (long i = 0; i < n; i++) y[i] = awfl::rcp(x[i]);
 }
+extern "C" void probe_pow_gamma_near(const double* x, const double* B, const double* Bh, const double* Bl,
+                                     double* y, long n) {
+  for (long i = 0; i < n; i++) y[i] = awfl::pow_gamma_near(x[i], (x[i] - B[i]) / B[i], Bh[i], Bl[i]);
+}
