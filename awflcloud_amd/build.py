@@ -32,6 +32,25 @@ def build_hip():
     os.replace(tmp, LIB)
 
 
+def build_driver():
+    """C++ driver over the header-only Spatial/Temporal adapter (include/awfl)."""
+    src = os.path.join(PKG, "driver", "driver.cpp")
+    exe = os.path.join(PKG, "awfl_driver")
+    run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+         "-I" + os.path.join(ROOT, "include"), src, "-o", exe, "-L" + PKG, "-lawfl_dycore",
+         "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+
+
+def build_adapter_test():
+    """Native test of the C++ adapter (run by tests/test_gpu_adapter.py)."""
+    src = os.path.join(ROOT, "tests", "native", "adapter_test.cpp")
+    exe = os.path.join(ROOT, "tests", "native", "adapter_test")
+    run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+         "-I" + os.path.join(ROOT, "include"), src, "-o", exe, "-L" + PKG, "-lawfl_dycore",
+         "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../awflcloud_amd",
+         "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+
+
 def build_oracle():
     run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -39,6 +58,8 @@ def build_oracle():
 def build_all():
     build_tables()
     build_hip()
+    build_driver()
+    build_adapter_test()
     build_oracle()
 
 

@@ -461,6 +461,8 @@ struct InitDev {
   int nmodes;
   const double* modes;
   double u0, v0;
+  double y2d, perturb_amp;
+  uint64_t perturb_seed;
   int igw;
   double xlen, ylen, zlen;
 };
@@ -499,7 +501,7 @@ __global__ void k_init(double* __restrict__ st, Grid G, InitDev S) {
       }
       for (int jj = 0; jj < ORD; jj++)
         for (int ii = 0; ii < ORD; ii++) {
-          double yloc = G.sim2d ? S.ylen / 2 : (j + 0.5) * G.dy + Gll<ORD>::pts[jj] * G.dy;
+          double yloc = G.sim2d ? (S.y2d != 0 ? S.y2d : S.ylen / 2) : (j + 0.5) * G.dy + Gll<ORD>::pts[jj] * G.dy;
           double xloc = (i + 0.5) * G.dx + Gll<ORD>::pts[ii] * G.dx;
           double wt = Gll<ORD>::wts[kk] * Gll<ORD>::wts[jj] * Gll<ORD>::wts[ii];
           double tp = 0;
@@ -507,6 +509,7 @@ __global__ void k_init(double* __restrict__ st, Grid G, InitDev S) {
             double xn = (xloc - S.x0[b]) / S.xr[b], yn = (yloc - S.y0[b]) / S.yr[b], zn = (zloc - S.z0[b]) / S.zr[b];
             double dist = sqrt(xn * xn + yn * yn + zn * zn);
             if (S.shape[b] == 0) tp += S.amp[b] * fmax(1. - dist, 0.);
+            else if (S.shape[b] == 2) tp += S.amp[b] * sin(M_PI * zloc / S.zlen) / sqrt(1 + xn * xn + yn * yn);
             else tp += dist <= 1. ? S.amp[b] * pow((cos(M_PI * dist) + 1) / 2, S.pwr[b]) : 0.;
           }
           tp += cell_tp;
@@ -515,6 +518,10 @@ __global__ void k_init(double* __restrict__ st, Grid G, InitDev S) {
           if (S.u0 != 0) sU += S.u0 * rh * wt;
           if (S.v0 != 0 && !G.sim2d) sV += S.v0 * rh * wt;
         }
+    }
+    if (S.perturb_amp != 0) {     // S:488-499 with a seeded generator
+      uint64_t h = splitmix64(splitmix64(S.perturb_seed) ^ (((uint64_t)k * G.ny + j) * G.nx + i));
+      sT *= 1. + (double)(h >> 11) * 0x1.0p-53 * S.perturb_amp;
     }
     st[0 * G.sl + c] = 0;
     st[1 * G.sl + c] = sU;
@@ -806,6 +813,7 @@ int awfl_init_state(awfl_ctx* c, double* d_state, const awfl_init_spec* s) {
   }
   D.noise_amp = s->noise_amp; D.noise_seed = s->noise_seed;
   D.u0 = s->u0; D.v0 = s->v0; D.igw = c->cfg.data_spec == 4;
+  D.y2d = s->y2d; D.perturb_amp = s->perturb_amp; D.perturb_seed = s->perturb_seed;
   D.xlen = c->cfg.xlen; D.ylen = c->cfg.ylen; D.zlen = c->cfg.zlen;
   double* d_modes = nullptr;
   if (s->nmodes > 0) {

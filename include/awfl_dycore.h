@@ -59,13 +59,19 @@ typedef struct {
  * theta' field evaluated at cell centres. */
 typedef struct {
   int nbub;
-  int shape[4];                       /* 0 linear cone, 1 cosine^pwr */
+  int shape[4];                       /* 0 linear cone, 1 cosine^pwr, 2 igw profile
+                                         amp sin(pi z/zlen)/sqrt(1+xn^2+yn^2) with
+                                         xn = (x-x0)/xr, yn = (y-y0)/yr (Profiles.h:76-82) */
   double x0[4], y0[4], z0[4], xr[4], yr[4], zr[4], amp[4], pwr[4];
   double noise_amp;                   /* theta' noise uniform in [-amp, amp) K */
   uint64_t noise_seed;
   int nmodes;                         /* modes: host array nmodes x (kx,ky,kz,amp,phase) */
   const double* modes;
   double u0, v0;                      /* uniform wind added as rho_h*u0 (igw uses 20) */
+  double y2d;                         /* y of every quadrature point when ny == 1
+                                         (0: ylen/2, S:437-441; igw uses ylen/3, S:469) */
+  double perturb_amp;                 /* (rho theta)' *= 1 + perturb_amp * U[0,1) per cell */
+  uint64_t perturb_seed;              /* (S:488-499 with a seeded generator) */
 } awfl_init_spec;
 
 /* Spatial::init (S:192-340): builds the transform tables (S:274-315) and the

@@ -476,7 +476,7 @@ void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* sp, double* 
             for (int ii = 0; ii < ord; ii++) {
               double zloc = (k + 0.5) * dz + c->gllPts_ord[kk] * dz;
               double yloc;
-              if (c->sim2d) yloc = ylen / 2;
+              if (c->sim2d) yloc = sp->y2d != 0 ? sp->y2d : ylen / 2;
               else yloc = (j + 0.5) * dy + c->gllPts_ord[jj] * dy;
               double xloc = (i + 0.5) * dx + c->gllPts_ord[ii] * dx;
               double wt = c->gllWts_ord[kk] * c->gllWts_ord[jj] * c->gllWts_ord[ii];
@@ -494,7 +494,10 @@ void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* sp, double* 
                 if (sp->shape[b] == 0)
                   tp += ellipsoid_linear(xloc, yloc, zloc, sp->x0[b], sp->y0[b], sp->z0[b], sp->xr[b], sp->yr[b],
                                          sp->zr[b], sp->amp[b]);
-                else
+                else if (sp->shape[b] == 2) {   /* Profiles.h:76-82 */
+                  double xn = (xloc - sp->x0[b]) / sp->xr[b], yn = (yloc - sp->y0[b]) / sp->yr[b];
+                  tp += sp->amp[b] * sin(M_PI * zloc / zlen) / sqrt(1 + xn * xn + yn * yn);
+                } else
                   tp += ellipsoid_cosine(xloc, yloc, zloc, sp->x0[b], sp->y0[b], sp->z0[b], sp->xr[b], sp->yr[b],
                                          sp->zr[b], sp->amp[b], sp->pwr[b]);
               }
@@ -504,7 +507,11 @@ void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* sp, double* 
               if (sp->u0 != 0) ST(idU, hs + k, hs + j, hs + i) += sp->u0 * rh * wt;
               if (sp->v0 != 0 && !c->sim2d) ST(idV, hs + k, hs + j, hs + i) += sp->v0 * rh * wt;
             }
-        (void)zlen;
+        if (sp->perturb_amp != 0) {
+          uint64_t h = splitmix64(splitmix64(sp->perturb_seed) ^ (((uint64_t)k * c->ny + j) * c->nx + i));
+          double rval = (double)(h >> 11) * 0x1.0p-53;
+          ST(idT, hs + k, hs + j, hs + i) *= 1. + rval * sp->perturb_amp;
+        }
       }
 }
 

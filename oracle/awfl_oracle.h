@@ -86,13 +86,17 @@ void oracle_riemann(const oracle_ctx* c, int dir, const double* qL, const double
 /* ---- initial data (S:428-485 plus the synthetic fields of the benchmarks) ---- */
 typedef struct {
   int nbub;                      /* number of ellipsoids (<= 4) */
-  int shape[4];                  /* 0 linear cone (Profiles.h:49-58), 1 cosine (Profiles.h:60-73) */
+  int shape[4];                  /* 0 linear cone (Profiles.h:49-58), 1 cosine (Profiles.h:60-73),
+                                    2 igw profile (Profiles.h:76-82) with hr = xr = yr */
   double x0[4], y0[4], z0[4], xr[4], yr[4], zr[4], amp[4], pwr[4];
   double noise_amp;              /* uniform per-cell theta' noise in [-amp, amp) */
   uint64_t noise_seed;
   int nmodes;                    /* band-limited theta' field evaluated at cell centres */
   const double* modes;           /* nmodes x (kx, ky, kz, amp, phase) */
   double u0, v0;                 /* uniform background wind (m/s), added as rho_h*u0 */
+  double y2d;                    /* y of the quadrature points when ny == 1 (0: ylen/2) */
+  double perturb_amp;            /* (rho theta)' *= 1 + amp * U[0,1) (S:488-499, seeded) */
+  uint64_t perturb_seed;
 } oracle_init_spec;
 void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* s, double* state);
 

@@ -27,7 +27,8 @@ class OracleInitSpec(C.Structure):
                 ("amp", C.c_double * 4), ("pwr", C.c_double * 4),
                 ("noise_amp", C.c_double), ("noise_seed", C.c_uint64),
                 ("nmodes", C.c_int), ("modes", C.POINTER(C.c_double)),
-                ("u0", C.c_double), ("v0", C.c_double)]
+                ("u0", C.c_double), ("v0", C.c_double),
+                ("y2d", C.c_double), ("perturb_amp", C.c_double), ("perturb_seed", C.c_uint64)]
 
 
 _lib = None
@@ -74,7 +75,8 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def make_spec(bubbles=(), noise_amp=0.0, noise_seed=0, modes=None, u0=0.0, v0=0.0):
+def make_spec(bubbles=(), noise_amp=0.0, noise_seed=0, modes=None, u0=0.0, v0=0.0, y2d=0.0, perturb_amp=0.0,
+              perturb_seed=0):
     """bubbles: iterable of dicts(shape, x0, y0, z0, xr, yr, zr, amp, pwr)."""
     s = OracleInitSpec()
     bubbles = list(bubbles)
@@ -94,6 +96,7 @@ def make_spec(bubbles=(), noise_amp=0.0, noise_seed=0, modes=None, u0=0.0, v0=0.
         s.nmodes = m.shape[0]
         s.modes = _dp(m)
     s.u0, s.v0 = u0, v0
+    s.y2d, s.perturb_amp, s.perturb_seed = y2d, perturb_amp, perturb_seed
     return s
 
 
