@@ -12,8 +12,10 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libawfl_dycore.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 arithmetic stays off in every HIP build of this package.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# translation units of libawfl_dycore.so: FP64 sweeps + ABI, FP32 perturbation form
+HIP_UNITS = ["awfl_dycore.hip", "awfl_pert.hip"]
 
 
 def run(cmd, **kw):
@@ -26,9 +28,21 @@ def build_tables():
 
 
 def build_hip():
-    src = os.path.join(CSRC, "awfl_dycore.hip")
+    """Compile the units in parallel, then link one shared library."""
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs, objs = [], []
+    for u in HIP_UNITS:
+        obj = os.path.join(objdir, os.path.splitext(u)[0] + ".o")
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, u), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((u, subprocess.Popen(cmd)))
+        objs.append(obj)
+    failed = [u for u, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError("hipcc failed for " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    run([HIPCC] + HIP_FLAGS + [src, "-o", tmp])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp])
     os.replace(tmp, LIB)
 
 

@@ -18,10 +18,17 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "awfl_fastmath.hpp"
 #include "awfl_tables.hpp"
 
 namespace awfl {
+
+// reciprocal of a positive normal value: refined FP64 (rcp) or the FP32
+// hardware estimate (~1 ulp)
+__device__ __forceinline__ double rcp_t(double x) { return rcp(x); }
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // phys_params.h:4-10
 constexpr double GRAV = 9.81;
@@ -35,14 +42,14 @@ enum { idR = 0, idU = 1, idV = 2, idW = 3, idT = 4 };
 // truncated total-variation indicator: the generated quadratic form of
 // coefs_to_tv<N>, terms summed left to right
 // ---------------------------------------------------------------------------
-template <int N, int M>
-__device__ __forceinline__ double tv_eval(const double (&a)[M]) {
-  double r = 0.0;
+template <int N, int M, class T>
+__device__ __forceinline__ T tv_eval(const T (&a)[M]) {
+  T r = 0;
   sfor<Tv<N>::n>([&](auto t) {
     constexpr int j = Tv<N>::j[t];
     constexpr int k = Tv<N>::k[t];
-    constexpr double c = Tv<N>::c[t];
-    double term;
+    constexpr T c = (T)Tv<N>::c[t];
+    T term;
     if constexpr (j == k) term = c * (a[j] * a[j]);
     else term = c * a[j] * a[k];
     if constexpr (t == 0) r = term;
@@ -52,9 +59,9 @@ __device__ __forceinline__ double tv_eval(const double (&a)[M]) {
 }
 
 // ex[i] = prod_{j != i} b[j] and P = prod_j b[j] (prefix/suffix products)
-template <int N>
-__device__ __forceinline__ void excl_products(const double (&b)[N], double (&ex)[N], double& P) {
-  double pre[N], suf[N];
+template <int N, class T>
+__device__ __forceinline__ void excl_products(const T (&b)[N], T (&ex)[N], T& P) {
+  T pre[N], suf[N];
   pre[0] = 1.0;
   sfor<N - 1>([&](auto i) {
     if constexpr (i == 0) pre[1] = b[0];
@@ -77,81 +84,99 @@ __device__ __forceinline__ void excl_products(const double (&b)[N], double (&ex)
 // ---------------------------------------------------------------------------
 // weno::compute_weno_coefs: ord cell averages u -> ord polynomial coefficients
 // ---------------------------------------------------------------------------
-template <int ORD>
-__device__ __forceinline__ void weno_coefs(const double (&u)[ORD], double (&aw)[ORD]) {
+template <int ORD, class T>
+__device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[ORD]) {
   using W = Weno<ORD>;
   constexpr int hs = W::hs;
-  double a[hs + 2][ORD];
+  T a[hs + 2][ORD];
   // hs+1 degree-hs candidates on sub-stencils i..i+hs (WenoLimiter.h:110-116)
   sfor<hs + 1>([&](auto i) {
     sfor<hs + 1>([&](auto ii) {
-      double t = -0.0;
+      T t = -0.0;
       sfor<hs + 1>([&](auto s) {
-        constexpr double cf = W::recon[i][s][ii];
-        if constexpr (cf != 0.0) t += cf * u[i + s];
+        constexpr T cf = (T)W::recon[i][s][ii];
+        if constexpr (cf != 0) t += cf * u[i + s];
       });
       a[i][ii] = t;
     });
   });
   // order-ORD polynomial on the full stencil (:117-121)
   sfor<ORD>([&](auto ii) {
-    double t = -0.0;
+    T t = -0.0;
     sfor<ORD>([&](auto s) {
-      constexpr double cf = W::recon[hs + 1][s][ii];
-      if constexpr (cf != 0.0) t += cf * u[s];
+      constexpr T cf = (T)W::recon[hs + 1][s][ii];
+      if constexpr (cf != 0) t += cf * u[s];
     });
     a[hs + 1][ii] = t;
   });
   // "bridge" polynomial (:123-131)
   sfor<hs + 1>([&](auto i) {
-    sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= W::idl[i] * a[i][ii]; });
+    sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= (T)W::idl[i] * a[i][ii]; });
   });
-  constexpr double ridl = 1.0 / W::idl[hs + 1];
+  constexpr T ridl = (T)(1.0 / W::idl[hs + 1]);
   sfor<ORD>([&](auto ii) { a[hs + 1][ii] *= ridl; });
   // total variation of every candidate (:133-143), sigma blend (:145-151)
-  double tv[hs + 2];
+  T tv[hs + 2];
   sfor<hs + 1>([&](auto i) { tv[i] = tv_eval<hs + 1>(a[i]); });
   tv[hs + 1] = tv_eval<ORD>(a[hs + 1]);
-  double lo_avg = 0.0;
+  T lo_avg = 0;
   sfor<hs + 1>([&](auto i) { lo_avg += tv[i]; });
-  lo_avg *= 1.0 / (hs + 1);
-  tv[hs + 1] = lo_avg + (tv[hs + 1] - lo_avg) * W::sigma;
+  lo_avg *= (T)(1.0 / (hs + 1));
+  tv[hs + 1] = lo_avg + (tv[hs + 1] - lo_avg) * (T)W::sigma;
   // weights ~ idl / TV^2, convexify, map, convexify (:153-161).  Each
   // normalisation w_i = (a_i / b_i) / (sum_k a_k / b_k + eps) is formed over
   // the common denominator: a_i prod_{j!=i} b_j / (sum_k a_k prod_{j!=k} b_j
   // + eps prod_j b_j), one reciprocal instead of hs+3 divisions.
-  double w[hs + 2];
-  {
-    double D[hs + 2], ex[hs + 2], P;
-    sfor<hs + 2>([&](auto i) { D[i] = tv[i] * tv[i] + WENO_EPS; });
-    excl_products<hs + 2>(D, ex, P);
-    double N = 0.0;
-    sfor<hs + 2>([&](auto i) {
-      w[i] = W::idl[i] * ex[i];
-      N += w[i];
-    });
-    double rs = rcp(fma(WENO_EPS, P, N));
+  T w[hs + 2];
+  if constexpr (std::is_same<T, double>::value) {
+    {
+      T D[hs + 2], ex[hs + 2], P;
+      sfor<hs + 2>([&](auto i) { D[i] = tv[i] * tv[i] + WENO_EPS; });
+      excl_products<hs + 2>(D, ex, P);
+      T N = 0.0;
+      sfor<hs + 2>([&](auto i) {
+        w[i] = W::idl[i] * ex[i];
+        N += w[i];
+      });
+      T rs = rcp(fma(WENO_EPS, P, N));
+      sfor<hs + 2>([&](auto i) { w[i] *= rs; });
+    }
+    {
+      T A[hs + 2], E[hs + 2], ex[hs + 2], P;
+      sfor<hs + 2>([&](auto i) {
+        constexpr double d = W::idl[i];
+        A[i] = w[i] * (d + d * d - 3.0 * d * w[i] + w[i] * w[i]);
+        E[i] = d * d + w[i] * (1.0 - 2.0 * d);
+      });
+      excl_products<hs + 2>(E, ex, P);
+      T M = 0.0;
+      sfor<hs + 2>([&](auto i) {
+        w[i] = A[i] * ex[i];
+        M += w[i];
+      });
+      T rs = rcp(fma(WENO_EPS, P, M));
+      sfor<hs + 2>([&](auto i) { w[i] *= rs; });
+    }
+  } else {
+    // FP32: products of eps-sized denominators would underflow; one hardware
+    // reciprocal per weight is cheap in FP32
+    sfor<hs + 2>([&](auto i) { w[i] = (T)W::idl[i] * rcp_t(tv[i] * tv[i] + (T)WENO_EPS); });
+    T sum = 0;
+    sfor<hs + 2>([&](auto i) { sum += w[i]; });
+    T rs = rcp_t(sum + (T)WENO_EPS);
     sfor<hs + 2>([&](auto i) { w[i] *= rs; });
-  }
-  {
-    double A[hs + 2], E[hs + 2], ex[hs + 2], P;
     sfor<hs + 2>([&](auto i) {
-      constexpr double d = W::idl[i];
-      A[i] = w[i] * (d + d * d - 3.0 * d * w[i] + w[i] * w[i]);
-      E[i] = d * d + w[i] * (1.0 - 2.0 * d);
+      constexpr T d = (T)W::idl[i];
+      w[i] = w[i] * (d + d * d - (T)3 * d * w[i] + w[i] * w[i]) * rcp_t(d * d + w[i] * ((T)1 - (T)2 * d));
     });
-    excl_products<hs + 2>(E, ex, P);
-    double M = 0.0;
-    sfor<hs + 2>([&](auto i) {
-      w[i] = A[i] * ex[i];
-      M += w[i];
-    });
-    double rs = rcp(fma(WENO_EPS, P, M));
+    sum = 0;
+    sfor<hs + 2>([&](auto i) { sum += w[i]; });
+    rs = rcp_t(sum + (T)WENO_EPS);
     sfor<hs + 2>([&](auto i) { w[i] *= rs; });
   }
   // weighted sum (:163-171); candidates have no coefficients above hs
   sfor<ORD>([&](auto ii) {
-    double t = -0.0;
+    T t = -0.0;
     sfor<hs + 2>([&](auto i) {
       if constexpr (i == hs + 1 || ii <= hs) t += w[i] * a[i][ii];
     });
@@ -160,26 +185,26 @@ __device__ __forceinline__ void weno_coefs(const double (&u)[ORD], double (&aw)[
 }
 
 // reconstruct_gll_values (S:1800-1833): ord stencil -> NGLL point values
-template <int ORD, int NGLL>
-__device__ __forceinline__ void reconstruct(const double (&u)[ORD], bool doweno, double (&g)[NGLL]) {
+template <int ORD, int NGLL, class T>
+__device__ __forceinline__ void reconstruct(const T (&u)[ORD], bool doweno, T (&g)[NGLL]) {
   using P = Proj<ORD, NGLL>;
   if (doweno) {
-    double aw[ORD];
+    T aw[ORD];
     weno_coefs<ORD>(u, aw);
     sfor<NGLL>([&](auto ii) {
-      double t = -0.0;
+      T t = -0.0;
       sfor<ORD>([&](auto s) {
-        constexpr double cf = P::c2g[s][ii];
-        if constexpr (cf != 0.0) t += cf * aw[s];
+        constexpr T cf = (T)P::c2g[s][ii];
+        if constexpr (cf != 0) t += cf * aw[s];
       });
       g[ii] = t;
     });
   } else {
     sfor<NGLL>([&](auto ii) {
-      double t = -0.0;
+      T t = -0.0;
       sfor<ORD>([&](auto s) {
-        constexpr double cf = P::s2g[s][ii];
-        if constexpr (cf != 0.0) t += cf * u[s];
+        constexpr T cf = (T)P::s2g[s][ii];
+        if constexpr (cf != 0) t += cf * u[s];
       });
       g[ii] = t;
     });

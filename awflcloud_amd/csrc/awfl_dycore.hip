@@ -28,30 +28,11 @@
 
 #include "../../include/awfl_dycore.h"
 #include "awfl_cell.hpp"
+#include "awfl_internal.hpp"
 
 using namespace awfl;
 
 namespace {
-
-// ---------------------------------------------------------------------------
-// grid description passed to kernels by value
-// ---------------------------------------------------------------------------
-struct Grid {
-  int nx, ny, nz, hs;
-  int64_t sy, sz, sl;            // strides of the padded state (x stride 1)
-  int bc[3];
-  int weno_s, weno_w, sim2d;
-  double dx, dy, dz;
-  const double* hyDensCells;     // nz+2hs
-  const double* hyDensThetaCells;
-  const double* hyDensGLL;       // nz*ngll
-  const double* hyDensThetaGLL;
-  const double* hyPressureGLL;
-  const PowRef* prT;             // [nz]   x/y: B = hyDensThetaCells(hs+k)
-  const PowRef* prRT;            // [nz]   y faces: B = hyDensCells * hyDensThetaCells
-  const PowRef* prG;             // [nz*ngll] z: B = hyDensThetaGLL(k, kk)
-  const PowRef* prF;             // [nz+1] z faces: B at the face
-};
 
 __device__ __forceinline__ int64_t tend_index(const Grid& G, int l, int k, int j, int i) {
   return (((int64_t)l * G.nz + k) * G.ny + j) * G.nx + i;
@@ -557,75 +538,13 @@ const Variant* find_variant(int ord, int ngll) {
   return nullptr;
 }
 
-thread_local std::string g_create_error;
-
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// context
-// ---------------------------------------------------------------------------
-struct awfl_ctx {
-  awfl_config cfg{};
-  const Variant* var = nullptr;
-  int hs = 0;
-  bool sim2d = false;
-  double dx = 0, dy = 0, dz = 0;
-  hipStream_t stream = nullptr;
-  int dimSwitch = 1;
-  double dtInit = 0;
-  int device = 0;
-  std::vector<double> hy[8];
-  double* d_hy = nullptr;
-  PowRef* d_pref = nullptr;         // power references: prT | prRT | prG | prF
-  double* d_buf[2] = {nullptr, nullptr};
-  double* d_red = nullptr;          // reduction partials
-  int n_red = 0;
-  std::string err;
-  Grid grid() const;
-  int64_t state_elems() const {
-    return (int64_t)5 * (cfg.nz + 2 * hs) * (cfg.ny + 2 * hs) * (cfg.nx + 2 * hs);
-  }
-  int64_t tend_elems() const { return (int64_t)5 * cfg.nz * cfg.ny * cfg.nx; }
-};
+namespace awfl_detail {
+thread_local std::string g_create_error;
+}  // namespace awfl_detail
+using awfl_detail::g_create_error;
 
-Grid awfl_ctx::grid() const {
-  Grid G{};
-  G.nx = cfg.nx; G.ny = cfg.ny; G.nz = cfg.nz; G.hs = hs;
-  int64_t nxh = cfg.nx + 2 * hs, nyh = cfg.ny + 2 * hs, nzh = cfg.nz + 2 * hs;
-  G.sy = nxh; G.sz = nxh * nyh; G.sl = nxh * nyh * nzh;
-  G.bc[0] = cfg.bc_x; G.bc[1] = cfg.bc_y; G.bc[2] = cfg.bc_z;
-  G.weno_s = cfg.weno_scalars; G.weno_w = cfg.weno_winds; G.sim2d = sim2d;
-  G.dx = dx; G.dy = dy; G.dz = dz;
-  int nzh_i = (int)nzh, ng = cfg.nz * cfg.ngll;
-  G.hyDensCells = d_hy;
-  G.hyDensThetaCells = d_hy + nzh_i;
-  G.hyDensGLL = d_hy + 2 * nzh_i;
-  G.hyDensThetaGLL = d_hy + 2 * nzh_i + ng;
-  G.hyPressureGLL = d_hy + 2 * nzh_i + 2 * ng;
-  G.prT = d_pref;
-  G.prRT = d_pref + cfg.nz;
-  G.prG = d_pref + 2 * cfg.nz;
-  G.prF = d_pref + 2 * cfg.nz + ng;
-  return G;
-}
-
-static int fail(awfl_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  else g_create_error = msg;
-  return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                                       \
-  do {                                                                                           \
-    hipError_t e_ = (expr);                                                                      \
-    if (e_ != hipSuccess) return fail((ctx), AWFL_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-#define LAUNCH_CHECK(ctx)                                                                        \
-  do {                                                                                           \
-    hipError_t e_ = hipGetLastError();                                                           \
-    if (e_ != hipSuccess) return fail((ctx), AWFL_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
-  } while (0)
 
 // S:370-425: hydrostatic background, FP64 on the host
 static void build_background(awfl_ctx* c) {
@@ -709,9 +628,12 @@ int awfl_create(const awfl_config* cfg, awfl_ctx** out) {
   for (int b : {cfg->bc_x, cfg->bc_y, cfg->bc_z})
     if (b != AWFL_BC_PERIODIC && b != AWFL_BC_WALL) return fail(nullptr, AWFL_E_ARG, "bc must be periodic or wall");
   if (cfg->data_spec < 1 || cfg->data_spec > 4) return fail(nullptr, AWFL_E_ARG, "data_spec must be 1..4");
-  if (cfg->precision != AWFL_FP64) return fail(nullptr, AWFL_E_UNSUPPORTED, "only AWFL_FP64 is available");
+  if (cfg->precision != AWFL_FP64 && cfg->precision != AWFL_FP32_PERT)
+    return fail(nullptr, AWFL_E_ARG, "precision must be AWFL_FP64 or AWFL_FP32_PERT");
   const Variant* v = find_variant(cfg->ord, cfg->ngll);
   if (!v) return fail(nullptr, AWFL_E_UNSUPPORTED, "(ord, ngll) not compiled in");
+  if (cfg->precision == AWFL_FP32_PERT && !awfl_detail::has_f32_variant(cfg->ord, cfg->ngll))
+    return fail(nullptr, AWFL_E_UNSUPPORTED, "(ord, ngll) not compiled in for AWFL_FP32_PERT");
   int hs = (cfg->ord - 1) / 2;
   if (cfg->bc_x == AWFL_BC_PERIODIC && cfg->nx < hs) return fail(nullptr, AWFL_E_ARG, "nx < hs");
   int64_t lds = (int64_t)8 * (5 * (cfg->nx + 2 * hs) + 10 * cfg->nx);
@@ -760,6 +682,9 @@ void awfl_destroy(awfl_ctx* c) {
   if (c->d_pref) (void)hipFree(c->d_pref);
   if (c->d_red) (void)hipFree(c->d_red);
   for (double* b : c->d_buf)
+    if (b) (void)hipFree(b);
+  if (c->d_bgf) (void)hipFree(c->d_bgf);
+  for (float* b : c->f_buf)
     if (b) (void)hipFree(b);
   delete c;
 }
@@ -824,7 +749,7 @@ int awfl_init_state(awfl_ctx* c, double* d_state, const awfl_init_spec* s) {
   Grid G = c->grid();
   int64_t n = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
-  c->var->init<<<blocks, 256, 0, c->stream>>>(d_state, G, D);
+  static_cast<const Variant*>(c->var)->init<<<blocks, 256, 0, c->stream>>>(d_state, G, D);
   hipError_t e = hipGetLastError();
   if (d_modes) {
     hipError_t e2 = hipStreamSynchronize(c->stream);
@@ -910,7 +835,7 @@ static SweepK sweep_constants(const awfl_ctx* c, double dt, int dir) {
 static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {
   Grid G = c->grid();
   SweepK K = sweep_constants(c, dt, dir);
-  const Variant* v = c->var;
+  const Variant* v = static_cast<const Variant*>(c->var);
   if (dir == 0) {
     int nx = c->cfg.nx;
     int threads = std::min(256, (nx + 63) / 64 * 64);

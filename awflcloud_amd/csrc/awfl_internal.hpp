@@ -1,0 +1,115 @@
+// awfl_internal.hpp -- declarations shared by the translation units of the
+// library: the kernel grid description, the context behind awfl_ctx*, and
+// error helpers.  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/awfl_dycore.h"
+#include "awfl_cell.hpp"
+
+namespace awfl_detail {
+using awfl::PowRef;
+
+// ---------------------------------------------------------------------------
+// grid description passed to kernels by value
+// ---------------------------------------------------------------------------
+struct Grid {
+  int nx, ny, nz, hs;
+  int64_t sy, sz, sl;            // strides of the padded state (x stride 1)
+  int bc[3];
+  int weno_s, weno_w, sim2d;
+  double dx, dy, dz;
+  const double* hyDensCells;     // nz+2hs
+  const double* hyDensThetaCells;
+  const double* hyDensGLL;       // nz*ngll
+  const double* hyDensThetaGLL;
+  const double* hyPressureGLL;
+  const PowRef* prT;             // [nz]   x/y: B = hyDensThetaCells(hs+k)
+  const PowRef* prRT;            // [nz]   y faces: B = hyDensCells * hyDensThetaCells
+  const PowRef* prG;             // [nz*ngll] z: B = hyDensThetaGLL(k, kk)
+  const PowRef* prF;             // [nz+1] z faces: B at the face
+};
+
+extern thread_local std::string g_create_error;
+
+// true when the FP32 perturbation-form kernels exist for (ord, ngll)
+bool has_f32_variant(int ord, int ngll);
+
+}  // namespace awfl_detail
+
+using awfl_detail::Grid;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct awfl_ctx {
+  awfl_config cfg{};
+  const void* var = nullptr;       // FP64 kernel variant (awfl_dycore.hip)
+  int hs = 0;
+  bool sim2d = false;
+  double dx = 0, dy = 0, dz = 0;
+  hipStream_t stream = nullptr;
+  int dimSwitch = 1;
+  double dtInit = 0;
+  int device = 0;
+  std::vector<double> hy[8];
+  double* d_hy = nullptr;
+  awfl::PowRef* d_pref = nullptr;         // power references: prT | prRT | prG | prF
+  double* d_buf[2] = {nullptr, nullptr};
+  double* d_red = nullptr;          // reduction partials
+  // FP32 perturbation form (awfl_pert.hip): background tables and ping-pong
+  // buffers, created on first use
+  void* d_bgf = nullptr;
+  float* f_buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  int n_red = 0;
+  std::string err;
+  Grid grid() const;
+  int64_t state_elems() const {
+    return (int64_t)5 * (cfg.nz + 2 * hs) * (cfg.ny + 2 * hs) * (cfg.nx + 2 * hs);
+  }
+  int64_t tend_elems() const { return (int64_t)5 * cfg.nz * cfg.ny * cfg.nx; }
+};
+
+inline Grid awfl_ctx::grid() const {
+  Grid G{};
+  G.nx = cfg.nx; G.ny = cfg.ny; G.nz = cfg.nz; G.hs = hs;
+  int64_t nxh = cfg.nx + 2 * hs, nyh = cfg.ny + 2 * hs, nzh = cfg.nz + 2 * hs;
+  G.sy = nxh; G.sz = nxh * nyh; G.sl = nxh * nyh * nzh;
+  G.bc[0] = cfg.bc_x; G.bc[1] = cfg.bc_y; G.bc[2] = cfg.bc_z;
+  G.weno_s = cfg.weno_scalars; G.weno_w = cfg.weno_winds; G.sim2d = sim2d;
+  G.dx = dx; G.dy = dy; G.dz = dz;
+  int nzh_i = (int)nzh, ng = cfg.nz * cfg.ngll;
+  G.hyDensCells = d_hy;
+  G.hyDensThetaCells = d_hy + nzh_i;
+  G.hyDensGLL = d_hy + 2 * nzh_i;
+  G.hyDensThetaGLL = d_hy + 2 * nzh_i + ng;
+  G.hyPressureGLL = d_hy + 2 * nzh_i + 2 * ng;
+  G.prT = d_pref;
+  G.prRT = d_pref + cfg.nz;
+  G.prG = d_pref + 2 * cfg.nz;
+  G.prF = d_pref + 2 * cfg.nz + ng;
+  return G;
+}
+
+inline int fail(awfl_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else awfl_detail::g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                       \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail((ctx), AWFL_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_CHECK(ctx)                                                                        \
+  do {                                                                                           \
+    hipError_t e_ = hipGetLastError();                                                           \
+    if (e_ != hipSuccess) return fail((ctx), AWFL_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+

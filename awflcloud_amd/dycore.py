@@ -63,6 +63,28 @@ def bubble_spec(bubbles=(), noise_amp=0.0, noise_seed=0, modes=None, u0=0.0, v0=
     return s
 
 
+def band_limited_modes(seed, n=64, kmin=4.0, kmax=16.0, rms=0.2):
+    """Seeded band-limited theta' field as n Fourier modes (kx, ky, kz, amp, phase).
+
+    Integer wavevectors with kmin <= |k| <= kmax (distinct, kx > 0 so no two
+    modes alias), random phases and one common amplitude chosen so that the
+    field's mean square over the periodic box is rms**2 (each cosine
+    contributes amp**2 / 2)."""
+    rng = np.random.default_rng(seed)
+    lim = int(np.floor(kmax))
+    seen, rows = set(), []
+    while len(rows) < n:
+        k = rng.integers(-lim, lim + 1, size=3)
+        k[0] = abs(k[0])
+        if k[0] == 0 or not (kmin <= np.sqrt((k * k).sum()) <= kmax) or tuple(k) in seen:
+            continue
+        seen.add(tuple(k))
+        rows.append([float(k[0]), float(k[1]), float(k[2]), 0.0, rng.uniform(0.0, 2.0 * np.pi)])
+    m = np.array(rows)
+    m[:, 3] = rms / np.sqrt(n / 2.0)
+    return m
+
+
 _lib = None
 
 
@@ -93,6 +115,10 @@ def lib():
         L.awfl_get_background.argtypes = [P, C.c_int, C.POINTER(D), I64]
         L.awfl_state_sums.argtypes = [P, P, C.POINTER(D)]
         L.awfl_eos_pressure.argtypes = [P, P, P, I64]
+        L.awfl_state_to_f32.argtypes = [P, P, P, P]
+        L.awfl_state_from_f32.argtypes = [P, P, P, P]
+        L.awfl_sweep_f32.argtypes = [P, P, P, P, P, D, C.c_int]
+        L.awfl_time_step_f32.argtypes = [P, P, P, D]
         _lib = L
     return _lib
 
@@ -242,6 +268,41 @@ class Dycore:
         p = self.torch.empty_like(rt)
         self._check(self.L.awfl_eos_pressure(self.h, self._ptr(rt), self._ptr(p), rt.numel()))
         return p
+
+    # --- FP32 perturbation form: a state is a pair (s, c) holding s - c ---
+    def _chk_f32(self, t):
+        if t.dtype != self.torch.float32 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError("FP32 state must be a contiguous float32 device tensor")
+        if tuple(t.shape) != self.state_shape:
+            raise ValueError("FP32 state shape %s != %s" % (tuple(t.shape), self.state_shape))
+
+    def to_f32(self, state):
+        """FP64 state -> (s, c), float32 tensors with s - c = state to ~2^-48."""
+        self._chk_state(state)
+        s = self.torch.empty(self.state_shape, dtype=self.torch.float32, device="cuda")
+        c = self.torch.empty_like(s)
+        self._check(self.L.awfl_state_to_f32(self.h, self._ptr(state), self._ptr(s), self._ptr(c)))
+        return s, c
+
+    def from_f32(self, s, c, state=None):
+        self._chk_f32(s)
+        self._chk_f32(c)
+        if state is None:
+            state = self.new_state()
+        self._chk_state(state)
+        self._check(self.L.awfl_state_from_f32(self.h, self._ptr(s), self._ptr(c), self._ptr(state)))
+        return state
+
+    def time_step_f32(self, s, c, dt):
+        self._chk_f32(s)
+        self._chk_f32(c)
+        self._check(self.L.awfl_time_step_f32(self.h, self._ptr(s), self._ptr(c), dt))
+
+    def sweep_f32(self, s_in, c_in, s_out, c_out, dt, direction):
+        for t in (s_in, c_in, s_out, c_out):
+            self._chk_f32(t)
+        self._check(self.L.awfl_sweep_f32(self.h, self._ptr(s_in), self._ptr(c_in), self._ptr(s_out),
+                                          self._ptr(c_out), dt, direction))
 
     def state_sums(self, state):
         self._chk_state(state)

@@ -133,6 +133,32 @@ int awfl_eos_pressure(awfl_ctx* ctx, const double* d_rt, double* d_p, int64_t n)
  * (conservation diagnostics).  h_out[5]. Synchronises. */
 int awfl_state_sums(awfl_ctx* ctx, const double* d_state, double* h_out);
 
+/* ---- FP32 perturbation form -------------------------------------------------
+ * The state in FP32: two arrays (s, c) in the FP64 state's padded layout, the
+ * value held being s - c (c is the Kahan compensation of the updates).  The
+ * numerics are those of the FP64 split sweeps (S:1004-1707, 1713-2197 driven
+ * by T:42-67) rewritten on deviations from the hydrostatic background, FP32
+ * throughout on the device.  Available for (ord, ngll) = (3, 2) and (5, 3);
+ * usable on any context whose (ord, ngll) has them (a context created with
+ * precision AWFL_FP32_PERT checks that at creation).  No counterpart in the
+ * reference, which has only FP64 (const.h:14 typedef double real). */
+
+/* FP64 state -> (s, c) with s = (float)x, c = (float)(s - x), so that s - c
+ * carries x to about 2^-48 relative. */
+int awfl_state_to_f32(awfl_ctx* ctx, const double* d_state, float* d_s, float* d_c);
+
+/* (s, c) -> FP64 state x = (double)s - (double)c. */
+int awfl_state_from_f32(awfl_ctx* ctx, const float* d_s, const float* d_c, double* d_state);
+
+/* One FP32 split sweep out of place: (s_out, c_out) = (s_in, c_in) after
+ * the dir sweep with time step dt, the analogue of awfl_sweep. */
+int awfl_sweep_f32(awfl_ctx* ctx, const float* d_s_in, const float* d_c_in, float* d_s_out, float* d_c_out,
+                   double dt, int dir);
+
+/* One FP32 time step in place on (s, c): the three splits in the order of
+ * awfl_time_step, sharing the context's dimSwitch (T:42-67). */
+int awfl_time_step_f32(awfl_ctx* ctx, float* d_s, float* d_c, double dt);
+
 #ifdef __cplusplus
 }
 #endif

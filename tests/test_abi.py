@@ -42,7 +42,8 @@ def test_invalid_configurations_are_rejected():
     base = dict(nx=8, ny=8, nz=8, xlen=1.0, ylen=1.0, zlen=1.0, bc_x=0, bc_y=0, bc_z=1, weno_scalars=1,
                 weno_winds=1, ord=5, ngll=3, data_spec=1, precision=0)
     for change, code in [(dict(nx=0), -1), (dict(ord=4), -4), (dict(ngll=7), -4), (dict(bc_x=3), -1),
-                         (dict(xlen=0.0), -1), (dict(data_spec=9), -1), (dict(precision=5), -4)]:
+                         (dict(xlen=0.0), -1), (dict(data_spec=9), -1), (dict(precision=5), -1),
+                         (dict(ord=9, ngll=5, precision=1), -4)]:
         cfg = AwflConfig(**{**base, **change})
         rc = L.awfl_create(C.byref(cfg), C.byref(h))
         assert rc == code, (change, rc)
