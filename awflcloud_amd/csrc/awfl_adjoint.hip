@@ -1,0 +1,614 @@
+// awfl_adjoint.hip -- tangent-linear and discrete-adjoint FP64 split sweeps,
+// the adjoint of a time step, and checkpointed forward/adjoint runs (C ABI
+// awfl_sweep_tangent / awfl_sweep_adjoint / awfl_time_step_adjoint /
+// awfl_forward_ckpt / awfl_adjoint_ckpt in include/awfl_dycore.h).
+//
+// One split sweep is out = in + dt * T(in) (awfl_sweep).  Its adjoint
+// adj_in = adj_out + J^T (dt adj_out) runs as four kernels, one thread per
+// cell, over per-cell scratch in HBM:
+//   k_edges      cell forward: time-averaged edge states and z source   [11/cell]
+//   k_face_adj   face adjoints: flux adjoint dt/d (adj_out[right] -
+//                adj_out[left]) pulled back through the Riemann flux to the
+//                two edge states the face used                           [10/cell]
+//   k_cell_adj   cell reverse: recompute the cell's transform levels and
+//                pull the edge and source adjoints back to the stencil   [25/cell]
+//   k_gather     adj_in[c] = adj_out[c] + sum of the stencil adjoints of
+//                every cell whose stencil holds c (periodic wrap, or the
+//                wall clamp with its zeroed normal momentum)
+// The tangent-linear sweep evaluates the same forward with Dual numbers
+// (k_edges<Dual>, then k_tlm_update); it exists to test the adjoint by the
+// dot-product identity.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "awfl_adjoint.hpp"
+#include "awfl_internal.hpp"
+
+using namespace awfl;
+using awfl::adj::Dual;
+
+namespace {
+
+constexpr int NE = 11;          // edge record: eL[5], eR[5], z source
+constexpr int NEB = 10;         // edge adjoints: eLb[5], eRb[5]
+
+struct Pos {
+  int i, j, k;
+};
+
+__device__ __forceinline__ int64_t sidx(const Grid& G, int l, const Pos& p) {
+  return l * G.sl + (int64_t)(G.hs + p.k) * G.sz + (int64_t)(G.hs + p.j) * G.sy + G.hs + p.i;
+}
+__device__ __forceinline__ int64_t cidx(const Grid& G, const Pos& p) {
+  return ((int64_t)p.k * G.ny + p.j) * G.nx + p.i;
+}
+__device__ __forceinline__ int64_t ncell(const Grid& G) { return (int64_t)G.nx * G.ny * G.nz; }
+__device__ __forceinline__ Pos pos_of(const Grid& G, int64_t t) {
+  Pos p;
+  p.i = (int)(t % G.nx);
+  int64_t r = t / G.nx;
+  p.j = (int)(r % G.ny);
+  p.k = (int)(r / G.ny);
+  return p;
+}
+template <int DIR>
+__device__ __forceinline__ int along(const Pos& p) {
+  return DIR == 0 ? p.i : DIR == 1 ? p.j : p.k;
+}
+template <int DIR>
+__device__ __forceinline__ Pos moved(Pos p, int a) {
+  if (DIR == 0) p.i = a;
+  else if (DIR == 1) p.j = a;
+  else p.k = a;
+  return p;
+}
+template <int DIR>
+__device__ __forceinline__ int axis_len(const Grid& G) {
+  return DIR == 0 ? G.nx : DIR == 1 ? G.ny : G.nz;
+}
+// logical index a along an axis of n cells -> stored cell (periodic wrap or
+// wall clamp); zn marks a clamped slot, whose normal momentum is zero
+__device__ __forceinline__ int map_axis(int a, int n, bool per, bool& zn) {
+  zn = false;
+  if (per) {
+    int s = a % n;
+    return s < 0 ? s + n : s;
+  }
+  if (a < 0) {
+    zn = true;
+    return 0;
+  }
+  if (a >= n) {
+    zn = true;
+    return n - 1;
+  }
+  return a;
+}
+
+template <int NGLL, int DIR>
+__device__ __forceinline__ CellBg<NGLL> cell_bg(const Grid& G, int k) {
+  CellBg<NGLL> bg{};
+  if constexpr (DIR == 2) {
+    sfor<NGLL>([&](auto ii) {
+      bg.rG[ii] = G.hyDensGLL[k * NGLL + ii];
+      bg.rtG[ii] = G.hyDensThetaGLL[k * NGLL + ii];
+      bg.pG[ii] = G.hyPressureGLL[k * NGLL + ii];
+    });
+    const bool per = G.bc[2] == AWFL_BC_PERIODIC;
+    bg.wall_bot = !per && k == 0;
+    bg.wall_top = !per && k == G.nz - 1;
+  } else {
+    bg.r = G.hyDensCells[G.hs + k];
+    bg.rt = G.hyDensThetaCells[G.hs + k];
+    bg.wall_bot = bg.wall_top = false;
+  }
+  return bg;
+}
+
+// background pressure subtracted by the z flux at face f (S:1628-1629)
+template <int NGLL>
+__device__ __forceinline__ double face_pressure(const Grid& G, int f) {
+  return f < G.nz ? G.hyPressureGLL[f * NGLL] : G.hyPressureGLL[(G.nz - 1) * NGLL + NGLL - 1];
+}
+
+template <int ORD, int DIR>
+__device__ __forceinline__ void load_stencil(const double* __restrict__ st, const Grid& G, const Pos& p,
+                                             double (&w)[5][ORD]) {
+  constexpr int hs = (ORD - 1) / 2;
+  constexpr int NM = 1 + DIR;
+  const int n = axis_len<DIR>(G);
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const int a0 = along<DIR>(p);
+  sfor<ORD>([&](auto s) {
+    bool zn;
+    Pos q = moved<DIR>(p, map_axis(a0 - hs + s, n, per, zn));
+    sfor<5>([&](auto l) { w[l][s] = (zn && l == NM) ? 0.0 : st[sidx(G, l, q)]; });
+  });
+}
+
+template <int ORD, int DIR>
+__device__ __forceinline__ void load_stencil(const double* __restrict__ st, const double* __restrict__ dst,
+                                             const Grid& G, const Pos& p, Dual (&w)[5][ORD]) {
+  constexpr int hs = (ORD - 1) / 2;
+  constexpr int NM = 1 + DIR;
+  const int n = axis_len<DIR>(G);
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const int a0 = along<DIR>(p);
+  sfor<ORD>([&](auto s) {
+    bool zn;
+    Pos q = moved<DIR>(p, map_axis(a0 - hs + s, n, per, zn));
+    sfor<5>([&](auto l) {
+      const int64_t o = sidx(G, l, q);
+      w[l][s] = (zn && l == NM) ? Dual(0.0) : Dual(st[o], dst[o]);
+    });
+  });
+}
+
+// cell forward -> edge record (value; and tangent for Dual)
+template <int ORD, int NGLL, int DIR, class T>
+__global__ __launch_bounds__(64) void k_edges(const double* __restrict__ st, const double* __restrict__ dst, Grid G,
+                                              SweepK K, double* __restrict__ E, double* __restrict__ Ed) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nc = ncell(G);
+  if (t >= nc) return;
+  const Pos p = pos_of(G, t);
+  T w[5][ORD];
+  if constexpr (std::is_same<T, double>::value) load_stencil<ORD, DIR>(st, G, p, w);
+  else load_stencil<ORD, DIR>(st, dst, G, p, w);
+  const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, p.k);
+  adj::Tape<NGLL, T> tp;
+  T eL[5], eR[5], sa;
+  adj::cell_fwd<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, tp, eL, eR, sa);
+  sfor<5>([&](auto l) {
+    E[l * nc + t] = adj::val(eL[l]);
+    E[(5 + l) * nc + t] = adj::val(eR[l]);
+  });
+  E[10 * nc + t] = adj::val(sa);
+  if constexpr (!std::is_same<T, double>::value) {
+    sfor<5>([&](auto l) {
+      Ed[l * nc + t] = eL[l].d;
+      Ed[(5 + l) * nc + t] = eR[l].d;
+    });
+    Ed[10 * nc + t] = sa.d;
+  }
+}
+
+// the two edge states of face f (between cells f-1 and f along DIR) of the
+// line through p: interior/periodic (eR[f-1], eL[f]); wall faces take both
+// sides from the one adjacent cell
+template <int DIR>
+__device__ __forceinline__ void face_states(const double* __restrict__ E, const Grid& G, const Pos& p, int f,
+                                            int64_t& cl, int64_t& cr, int& offl, int& offr) {
+  const int n = axis_len<DIR>(G);
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  if (!per && f == 0) {
+    cl = cr = cidx(G, moved<DIR>(p, 0));
+    offl = offr = 0;                    // eL of cell 0 on both sides
+  } else if (!per && f == n) {
+    cl = cr = cidx(G, moved<DIR>(p, n - 1));
+    offl = offr = 5;                    // eR of cell n-1 on both sides
+  } else {
+    int a = f - 1;
+    if (a < 0) a += n;
+    int b = f == n ? 0 : f;
+    cl = cidx(G, moved<DIR>(p, a));
+    cr = cidx(G, moved<DIR>(p, b));
+    offl = 5;
+    offr = 0;
+  }
+}
+
+// Face adjoints.  Thread of cell p handles the face at its low side, and the
+// last cell of a wall line also the face at its high side.
+template <int NGLL, int DIR>
+__global__ __launch_bounds__(64) void k_face_adj(const double* __restrict__ E, const double* __restrict__ adj_out,
+                                                 Grid G, SweepK K, double* __restrict__ EB) {
+  constexpr int NM = 1 + DIR;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nc = ncell(G);
+  if (t >= nc) return;
+  const Pos p = pos_of(G, t);
+  const int n = axis_len<DIR>(G);
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const int a = along<DIR>(p);
+  const double c = K.dt * K.rd;
+  auto do_face = [&](int f) {
+    int64_t cl, cr;
+    int offl, offr;
+    face_states<DIR>(E, G, p, f, cl, cr, offl, offr);
+    double L[5], R[5], Fb[5], Lb[5], Rb[5];
+    sfor<5>([&](auto l) {
+      L[l] = E[(offl + l) * nc + cl];
+      R[l] = E[(offr + l) * nc + cr];
+      Lb[l] = Rb[l] = 0.0;
+    });
+    // out[left] -= c F, out[right] += c F (the cell that lies on each side)
+    const bool has_l = per || f > 0, has_r = per || f < n;
+    Pos pl = moved<DIR>(p, f == 0 ? n - 1 : f - 1), pr = moved<DIR>(p, f == n ? 0 : f);
+    sfor<5>([&](auto l) {
+      double v = 0.0;
+      if (has_r) v += adj_out[sidx(G, l, pr)];
+      if (has_l) v -= adj_out[sidx(G, l, pl)];
+      Fb[l] = (G.sim2d && l == idV) ? 0.0 : c * v;
+    });
+    adj::riemann_adj<DIR>(L, R, Fb, Lb, Rb);
+    if (!per && f == 0) {
+      sfor<5>([&](auto l) { EB[l * nc + cr] = Lb[l] + Rb[l]; });
+    } else if (!per && f == n) {
+      sfor<5>([&](auto l) { EB[(5 + l) * nc + cl] = Lb[l] + Rb[l]; });
+    } else {
+      sfor<5>([&](auto l) {
+        EB[(5 + l) * nc + cl] = Lb[l];
+        EB[l * nc + cr] = Rb[l];
+      });
+    }
+    (void)NM;
+  };
+  do_face(a);
+  if (!per && a == n - 1) do_face(n);
+}
+
+// Cell reverse: edge and source adjoints -> stencil adjoints STB[l*ORD+s][cell]
+template <int ORD, int NGLL, int DIR>
+__global__ __launch_bounds__(64) void k_cell_adj(const double* __restrict__ st, const double* __restrict__ EB,
+                                                 const double* __restrict__ adj_out, Grid G, SweepK K,
+                                                 double* __restrict__ STB) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nc = ncell(G);
+  if (t >= nc) return;
+  const Pos p = pos_of(G, t);
+  double w[5][ORD];
+  load_stencil<ORD, DIR>(st, G, p, w);
+  const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, p.k);
+  double eLb[5], eRb[5];
+  sfor<5>([&](auto l) {
+    eLb[l] = EB[l * nc + t];
+    eRb[l] = EB[(5 + l) * nc + t];
+  });
+  // z: the source enters rho*w as out += dt * src
+  const double srcb = DIR == 2 ? K.dt * adj_out[sidx(G, idW, p)] : 0.0;
+  double stb[5][ORD];
+  sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { stb[l][s] = 0.0; }); });
+  adj::cell_adj<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eLb, eRb, srcb, stb);
+  sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { STB[(l * ORD + s) * nc + t] = stb[l][s]; }); });
+}
+
+// adj_in over the padded layout: zero in the halo, in the interior the
+// identity part plus every stencil adjoint that points at the cell
+template <int ORD, int DIR>
+__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ STB, const double* __restrict__ adj_out,
+                                                Grid G, double* __restrict__ adj_in) {
+  constexpr int hs = (ORD - 1) / 2;
+  constexpr int NM = 1 + DIR;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nxh = G.nx + 2 * G.hs, nyh = G.ny + 2 * G.hs;
+  if (t >= G.sl) return;
+  const int ih = (int)(t % nxh);
+  const int jh = (int)((t / nxh) % nyh);
+  const int kh = (int)(t / (nxh * nyh));
+  const int i = ih - G.hs, j = jh - G.hs, k = kh - G.hs;
+  if (i < 0 || i >= G.nx || j < 0 || j >= G.ny || k < 0 || k >= G.nz) {
+    sfor<5>([&](auto l) { adj_in[l * G.sl + t] = 0.0; });
+    return;
+  }
+  const int64_t nc = ncell(G);
+  const Pos p{i, j, k};
+  const int n = axis_len<DIR>(G);
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const int a = along<DIR>(p);
+  double acc[5];
+  sfor<5>([&](auto l) { acc[l] = adj_out[l * G.sl + t]; });
+  sfor<ORD>([&](auto s) {
+    // the cell whose stencil slot s is this cell
+    int src = a + hs - s;
+    bool ok = true;
+    if (per) {
+      src %= n;
+      if (src < 0) src += n;
+    } else {
+      ok = src >= 0 && src < n;
+    }
+    if (ok) {
+      const int64_t c = cidx(G, moved<DIR>(p, src));
+      sfor<5>([&](auto l) { acc[l] += STB[(l * ORD + s) * nc + c]; });
+    }
+  });
+  if (!per) {
+    // clamped slots: below the first cell (cells 0..hs-1) and above the last
+    if (a == 0) {
+      sfor<hs>([&](auto cc) {
+        sfor<ORD>([&](auto s) {
+          if constexpr (cc - hs + s < 0) {
+            if (cc < n) {
+              const int64_t c = cidx(G, moved<DIR>(p, cc));
+              sfor<5>([&](auto l) {
+                if constexpr (l != NM) acc[l] += STB[(l * ORD + s) * nc + c];
+              });
+            }
+          }
+        });
+      });
+    }
+    if (a == n - 1) {
+      sfor<hs>([&](auto m) {
+        const int cc = n - 1 - m;       // cells n-hs..n-1
+        sfor<ORD>([&](auto s) {
+          // slot s of cell cc lies at cc - hs + s >= n  <=>  s >= hs + 1 + m
+          if constexpr (s >= hs + 1 + m) {
+            if (cc >= 0) {
+              const int64_t c = cidx(G, moved<DIR>(p, cc));
+              sfor<5>([&](auto l) {
+                if constexpr (l != NM) acc[l] += STB[(l * ORD + s) * nc + c];
+              });
+            }
+          }
+        });
+      });
+    }
+  }
+  sfor<5>([&](auto l) { adj_in[l * G.sl + t] = acc[l]; });
+}
+
+// tangent-linear update over the padded layout (halo zero)
+template <int NGLL, int DIR>
+__global__ __launch_bounds__(256) void k_tlm_update(const double* __restrict__ in, const double* __restrict__ din,
+                                                    const double* __restrict__ E, const double* __restrict__ Ed,
+                                                    Grid G, SweepK K, double* __restrict__ out,
+                                                    double* __restrict__ dout) {
+  constexpr int NM = 1 + DIR;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nxh = G.nx + 2 * G.hs, nyh = G.ny + 2 * G.hs;
+  if (t >= G.sl) return;
+  const int i = (int)(t % nxh) - G.hs;
+  const int j = (int)((t / nxh) % nyh) - G.hs;
+  const int k = (int)(t / (nxh * nyh)) - G.hs;
+  if (i < 0 || i >= G.nx || j < 0 || j >= G.ny || k < 0 || k >= G.nz) {
+    sfor<5>([&](auto l) { out[l * G.sl + t] = dout[l * G.sl + t] = 0.0; });
+    return;
+  }
+  const int64_t nc = ncell(G);
+  const Pos p{i, j, k};
+  const int a = along<DIR>(p);
+  auto flux = [&](int f, Dual (&F)[5]) {
+    int64_t cl, cr;
+    int offl, offr;
+    face_states<DIR>(E, G, p, f, cl, cr, offl, offr);
+    Dual L[5], R[5];
+    sfor<5>([&](auto l) {
+      L[l] = Dual(E[(offl + l) * nc + cl], Ed[(offl + l) * nc + cl]);
+      R[l] = Dual(E[(offr + l) * nc + cr], Ed[(offr + l) * nc + cr]);
+    });
+    adj::riemann_fwd<DIR>(L, R, DIR == 2 ? face_pressure<NGLL>(G, f) : 0.0, F);
+  };
+  Dual F0[5], F1[5];
+  flux(a, F0);
+  flux(a + 1, F1);
+  const int64_t c = cidx(G, p);
+  sfor<5>([&](auto l) {
+    Dual td = (F1[l] - F0[l]) * Dual(-K.rd);
+    if constexpr (DIR == 2 && l == idW) td = Dual(E[10 * nc + c], Ed[10 * nc + c]) + td;
+    if (G.sim2d && l == idV) td = Dual(0.0);
+    out[l * G.sl + t] = in[l * G.sl + t] + K.dt * td.v;
+    dout[l * G.sl + t] = din[l * G.sl + t] + K.dt * td.d;
+  });
+  (void)NM;
+}
+
+struct VariantA {
+  int ord, ngll;
+  void (*edges[3])(const double*, const double*, Grid, SweepK, double*, double*);
+  void (*edges_d[3])(const double*, const double*, Grid, SweepK, double*, double*);
+  void (*face[3])(const double*, const double*, Grid, SweepK, double*);
+  void (*cell[3])(const double*, const double*, const double*, Grid, SweepK, double*);
+  void (*gather[3])(const double*, const double*, Grid, double*);
+  void (*tlm[3])(const double*, const double*, const double*, const double*, Grid, SweepK, double*, double*);
+};
+
+#define AWFL_VARIANT_A(O, N)                                                                        \
+  VariantA {                                                                                        \
+    O, N, {k_edges<O, N, 0, double>, k_edges<O, N, 1, double>, k_edges<O, N, 2, double>},           \
+        {k_edges<O, N, 0, Dual>, k_edges<O, N, 1, Dual>, k_edges<O, N, 2, Dual>},                   \
+        {k_face_adj<N, 0>, k_face_adj<N, 1>, k_face_adj<N, 2>},                                     \
+        {k_cell_adj<O, N, 0>, k_cell_adj<O, N, 1>, k_cell_adj<O, N, 2>},                            \
+        {k_gather<O, 0>, k_gather<O, 1>, k_gather<O, 2>},                                           \
+        {k_tlm_update<N, 0>, k_tlm_update<N, 1>, k_tlm_update<N, 2>},                               \
+  }
+
+const VariantA kVariantsA[] = {AWFL_VARIANT_A(3, 2), AWFL_VARIANT_A(5, 3)};
+
+const VariantA* find_variant_a(int ord, int ngll) {
+  for (const VariantA& v : kVariantsA)
+    if (v.ord == ord && v.ngll == ngll) return &v;
+  return nullptr;
+}
+
+int ensure_scratch(awfl_ctx* c) {
+  if (!find_variant_a(c->cfg.ord, c->cfg.ngll))
+    return fail(c, AWFL_E_UNSUPPORTED, "adjoint: (ord, ngll) not compiled in");
+  const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  const int per_cell = NE + NEB + 5 * c->cfg.ord;
+  if (!c->a_scr) HIP_TRY(c, hipMalloc(&c->a_scr, sizeof(double) * nc * per_cell));
+  if (!c->a_tmp) HIP_TRY(c, hipMalloc(&c->a_tmp, sizeof(double) * c->state_elems()));
+  return AWFL_OK;
+}
+
+unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// adj_in = adj_out + J(in)^T (dt adj_out) for one sweep
+int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* adj_in, double dt, int dir) {
+  if (dir == 1 && c->sim2d) {
+    HIP_TRY(c, hipMemcpyAsync(adj_in, adj_out, sizeof(double) * c->state_elems(), hipMemcpyDeviceToDevice,
+                              c->stream));
+    return AWFL_OK;
+  }
+  int rc = ensure_scratch(c);
+  if (rc != AWFL_OK) return rc;
+  const VariantA* v = find_variant_a(c->cfg.ord, c->cfg.ngll);
+  const Grid G = c->grid();
+  const SweepK K = sweep_constants(c, dt, dir);
+  const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  double* E = c->a_scr;
+  double* EB = E + NE * nc;
+  double* STB = EB + NEB * nc;
+  v->edges[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(in, nullptr, G, K, E, nullptr);
+  LAUNCH_CHECK(c);
+  v->face[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(E, adj_out, G, K, EB);
+  LAUNCH_CHECK(c);
+  v->cell[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(in, EB, adj_out, G, K, STB);
+  LAUNCH_CHECK(c);
+  v->gather[dir]<<<blocks_for(G.sl, 256), 256, 0, c->stream>>>(STB, adj_out, G, adj_in);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
+int split_dirs(const awfl_ctx* c, int ds, int (&dirs)[3]) {
+  static const int fwd[3] = {0, 1, 2}, bwd[3] = {2, 1, 0};
+  int nd = 0;
+  for (int s = 0; s < 3; s++) {
+    int d = ds ? fwd[s] : bwd[s];
+    if (d == 1 && c->sim2d) continue;
+    dirs[nd++] = d;
+  }
+  return nd;
+}
+
+int ensure_states(awfl_ctx* c, int64_t nstates) {
+  if (c->a_states_n >= nstates) return AWFL_OK;
+  if (c->a_states) HIP_TRY(c, hipFree(c->a_states));
+  c->a_states = nullptr;
+  c->a_states_n = 0;
+  HIP_TRY(c, hipMalloc(&c->a_states, sizeof(double) * c->state_elems() * nstates));
+  c->a_states_n = nstates;
+  return AWFL_OK;
+}
+
+// reverse of the steps [n0, n1) whose split inputs are in c->a_states;
+// adj is updated in place (ping-pong through a_tmp)
+int reverse_steps(awfl_ctx* c, int n0, int n1, int ds0, double dt, double* adj) {
+  const int64_t el = c->state_elems();
+  double* cur = adj;
+  double* oth = c->a_tmp;
+  int dirs[3];
+  for (int n = n1 - 1; n >= n0; n--) {
+    const int ds = (n % 2 == 0) ? ds0 : !ds0;
+    const int nd = split_dirs(c, ds, dirs);
+    for (int k = nd - 1; k >= 0; k--) {
+      const double* in = c->a_states + (int64_t)((n - n0) * nd + k) * el;
+      int rc = sweep_adjoint(c, in, cur, oth, dt, dirs[k]);
+      if (rc != AWFL_OK) return rc;
+      std::swap(cur, oth);
+    }
+  }
+  if (cur != adj) HIP_TRY(c, hipMemcpyAsync(adj, cur, sizeof(double) * el, hipMemcpyDeviceToDevice, c->stream));
+  return AWFL_OK;
+}
+
+// split inputs of steps [n0, n1) from the state before step n0
+int record_steps(awfl_ctx* c, const double* start, int n0, int n1, int ds0, double dt) {
+  const int64_t el = c->state_elems();
+  int dirs[3];
+  const int nd = split_dirs(c, 1, dirs);
+  int rc = ensure_states(c, (int64_t)(n1 - n0) * nd);
+  if (rc != AWFL_OK) return rc;
+  HIP_TRY(c, hipMemcpyAsync(c->a_states, start, sizeof(double) * el, hipMemcpyDeviceToDevice, c->stream));
+  const int64_t last = (int64_t)(n1 - n0) * nd - 1;
+  for (int n = n0; n < n1; n++) {
+    const int ds = (n % 2 == 0) ? ds0 : !ds0;
+    split_dirs(c, ds, dirs);
+    for (int k = 0; k < nd; k++) {
+      const int64_t idx = (int64_t)(n - n0) * nd + k;
+      if (idx == last) break;
+      rc = awfl_sweep(c, c->a_states + idx * el, c->a_states + (idx + 1) * el, dt, dirs[k]);
+      if (rc != AWFL_OK) return rc;
+    }
+  }
+  return AWFL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int awfl_sweep_tangent(awfl_ctx* c, const double* d_in, const double* d_din, double* d_out, double* d_dout,
+                       double dt, int dir) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_in || !d_din || !d_out || !d_dout) return fail(c, AWFL_E_ARG, "null buffer");
+  if (d_in == d_out || d_din == d_dout) return fail(c, AWFL_E_ARG, "tangent sweeps run out of place");
+  if (dir < 0 || dir > 2) return fail(c, AWFL_E_ARG, "dir must be 0, 1 or 2");
+  if (dir == 1 && c->sim2d) {
+    HIP_TRY(c, hipMemcpyAsync(d_out, d_in, sizeof(double) * c->state_elems(), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(d_dout, d_din, sizeof(double) * c->state_elems(), hipMemcpyDeviceToDevice, c->stream));
+    return AWFL_OK;
+  }
+  int rc = ensure_scratch(c);
+  if (rc != AWFL_OK) return rc;
+  const VariantA* v = find_variant_a(c->cfg.ord, c->cfg.ngll);
+  const Grid G = c->grid();
+  const SweepK K = sweep_constants(c, dt, dir);
+  const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  double* E = c->a_scr;
+  double* Ed = E + NE * nc;            // fits: NE <= NEB + 5*ord
+  v->edges_d[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(d_in, d_din, G, K, E, Ed);
+  LAUNCH_CHECK(c);
+  v->tlm[dir]<<<blocks_for(G.sl, 256), 256, 0, c->stream>>>(d_in, d_din, E, Ed, G, K, d_out, d_dout);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
+int awfl_sweep_adjoint(awfl_ctx* c, const double* d_in, const double* d_adj_out, double* d_adj_in, double dt,
+                       int dir) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_in || !d_adj_out || !d_adj_in) return fail(c, AWFL_E_ARG, "null buffer");
+  if (d_adj_out == d_adj_in) return fail(c, AWFL_E_ARG, "adjoint sweeps run out of place");
+  if (dir < 0 || dir > 2) return fail(c, AWFL_E_ARG, "dir must be 0, 1 or 2");
+  return sweep_adjoint(c, d_in, d_adj_out, d_adj_in, dt, dir);
+}
+
+int awfl_time_step_adjoint(awfl_ctx* c, const double* d_state_before, double* d_adj, double dt, int dim_switch) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state_before || !d_adj) return fail(c, AWFL_E_ARG, "null buffer");
+  int rc = ensure_scratch(c);
+  if (rc != AWFL_OK) return rc;
+  rc = record_steps(c, d_state_before, 0, 1, dim_switch ? 1 : 0, dt);
+  if (rc != AWFL_OK) return rc;
+  return reverse_steps(c, 0, 1, dim_switch ? 1 : 0, dt, d_adj);
+}
+
+int awfl_forward_ckpt(awfl_ctx* c, double* d_state, double dt, int nsteps, int every, double* d_ckpt) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state || !d_ckpt) return fail(c, AWFL_E_ARG, "null buffer");
+  if (nsteps < 0 || every < 1) return fail(c, AWFL_E_ARG, "nsteps >= 0 and every >= 1");
+  const int64_t el = c->state_elems();
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0)
+      HIP_TRY(c, hipMemcpyAsync(d_ckpt + (int64_t)(n / every) * el, d_state, sizeof(double) * el,
+                                hipMemcpyDeviceToDevice, c->stream));
+    int rc = awfl_time_step(c, d_state, dt);
+    if (rc != AWFL_OK) return rc;
+  }
+  return AWFL_OK;
+}
+
+int awfl_adjoint_ckpt(awfl_ctx* c, const double* d_ckpt, double dt, int nsteps, int every, int dim_switch0,
+                      double* d_adj) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_ckpt || !d_adj) return fail(c, AWFL_E_ARG, "null buffer");
+  if (nsteps < 0 || every < 1) return fail(c, AWFL_E_ARG, "nsteps >= 0 and every >= 1");
+  int rc = ensure_scratch(c);
+  if (rc != AWFL_OK) return rc;
+  const int64_t el = c->state_elems();
+  const int ds0 = dim_switch0 ? 1 : 0;
+  const int nseg = (nsteps + every - 1) / every;
+  for (int m = nseg - 1; m >= 0; m--) {
+    const int n0 = m * every, n1 = std::min(nsteps, n0 + every);
+    rc = record_steps(c, d_ckpt + (int64_t)m * el, n0, n1, ds0, dt);
+    if (rc != AWFL_OK) return rc;
+    rc = reverse_steps(c, n0, n1, ds0, dt, d_adj);
+    if (rc != AWFL_OK) return rc;
+  }
+  return AWFL_OK;
+}
+
+}  // extern "C"

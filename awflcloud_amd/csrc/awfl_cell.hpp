@@ -16,19 +16,23 @@
 //   time averages ............ S:2130-2177
 //   Riemann fluxes ........... S:842-887 (x), 1202-1245 (y), 1583-1629 (z)
 #pragma once
-#include <hip/hip_runtime.h>
-
 #include <type_traits>
 
-#include "awfl_fastmath.hpp"
+#include "awfl_fastmath.hpp"   // HIP runtime, or host stand-ins for __host__/__device__
 #include "awfl_tables.hpp"
 
 namespace awfl {
 
 // reciprocal of a positive normal value: refined FP64 (rcp) or the FP32
 // hardware estimate (~1 ulp)
-__device__ __forceinline__ double rcp_t(double x) { return rcp(x); }
-__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__host__ __device__ __forceinline__ double rcp_t(double x) { return rcp(x); }
+__host__ __device__ __forceinline__ float rcp_t(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
 
 // phys_params.h:4-10
 constexpr double GRAV = 9.81;
@@ -43,7 +47,7 @@ enum { idR = 0, idU = 1, idV = 2, idW = 3, idT = 4 };
 // coefs_to_tv<N>, terms summed left to right
 // ---------------------------------------------------------------------------
 template <int N, int M, class T>
-__device__ __forceinline__ T tv_eval(const T (&a)[M]) {
+__host__ __device__ __forceinline__ T tv_eval(const T (&a)[M]) {
   T r = 0;
   sfor<Tv<N>::n>([&](auto t) {
     constexpr int j = Tv<N>::j[t];
@@ -60,7 +64,7 @@ __device__ __forceinline__ T tv_eval(const T (&a)[M]) {
 
 // ex[i] = prod_{j != i} b[j] and P = prod_j b[j] (prefix/suffix products)
 template <int N, class T>
-__device__ __forceinline__ void excl_products(const T (&b)[N], T (&ex)[N], T& P) {
+__host__ __device__ __forceinline__ void excl_products(const T (&b)[N], T (&ex)[N], T& P) {
   T pre[N], suf[N];
   pre[0] = 1.0;
   sfor<N - 1>([&](auto i) {
@@ -85,7 +89,7 @@ __device__ __forceinline__ void excl_products(const T (&b)[N], T (&ex)[N], T& P)
 // weno::compute_weno_coefs: ord cell averages u -> ord polynomial coefficients
 // ---------------------------------------------------------------------------
 template <int ORD, class T>
-__device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[ORD]) {
+__host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[ORD]) {
   using W = Weno<ORD>;
   constexpr int hs = W::hs;
   T a[hs + 2][ORD];
@@ -186,7 +190,7 @@ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[ORD]) {
 
 // reconstruct_gll_values (S:1800-1833): ord stencil -> NGLL point values
 template <int ORD, int NGLL, class T>
-__device__ __forceinline__ void reconstruct(const T (&u)[ORD], bool doweno, T (&g)[NGLL]) {
+__host__ __device__ __forceinline__ void reconstruct(const T (&u)[ORD], bool doweno, T (&g)[NGLL]) {
   using P = Proj<ORD, NGLL>;
   if (doweno) {
     T aw[ORD];
@@ -253,7 +257,7 @@ struct SweepK {
 // the time- and cell-averaged gravity source of rho*w).
 // ---------------------------------------------------------------------------
 template <int ORD, int NGLL, int DIR>
-__device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s,
+__host__ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s,
                                            bool weno_w, const SweepK& K, double (&eL)[5], double (&eR)[5],
                                            double& src_avg) {
   constexpr int NA = NGLL;             // nAder = ngll (Temporal_ader.h:8)
@@ -396,7 +400,7 @@ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const Cel
 // normal velocity test is strict (u > 0).
 // ---------------------------------------------------------------------------
 template <int DIR>
-__device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], const FaceRef& fr,
+__host__ __device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], const FaceRef& fr,
                                         double (&F)[5]) {
   constexpr int NM = 1 + DIR;
   // tangential momentum indices in the order the reference uses them

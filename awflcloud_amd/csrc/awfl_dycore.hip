@@ -686,6 +686,8 @@ void awfl_destroy(awfl_ctx* c) {
   if (c->d_bgf) (void)hipFree(c->d_bgf);
   for (float* b : c->f_buf)
     if (b) (void)hipFree(b);
+  for (double* b : {c->a_scr, c->a_tmp, c->a_states})
+    if (b) (void)hipFree(b);
   delete c;
 }
 
@@ -814,22 +816,6 @@ int awfl_state_sums(awfl_ctx* c, const double* d_state, double* h_out) {
 
 }  // extern "C"
 
-// launch constants of a sweep along dir with time step dt
-static SweepK sweep_constants(const awfl_ctx* c, double dt, int dir) {
-  SweepK K{};
-  double dsw = dir == 0 ? c->dx : dir == 1 ? c->dy : c->dz;
-  K.dt = dt;
-  K.rd = 1.0 / dsw;
-  double dtmult = dt;
-  for (int kt = 0; kt < KMAX; kt++) {
-    K.ndt[kt] = -1.0 / (dsw * (kt + 1.0));
-    if (kt >= 1) {
-      K.tav[kt] = dtmult / (kt + 1.0);
-      dtmult *= dt;
-    }
-  }
-  return K;
-}
 
 // launch one sweep; tend selects the tendency output (layout [5][nz][ny][nx])
 static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {

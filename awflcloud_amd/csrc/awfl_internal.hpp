@@ -65,6 +65,13 @@ struct awfl_ctx {
   // buffers, created on first use
   void* d_bgf = nullptr;
   float* f_buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  // adjoint (awfl_adjoint.hip): per-cell scratch (edge states, their
+  // adjoints, stencil adjoints), one adjoint ping-pong state, and the split
+  // inputs of a recomputed checkpoint segment
+  double* a_scr = nullptr;
+  double* a_tmp = nullptr;
+  double* a_states = nullptr;
+  int64_t a_states_n = 0;
   int n_red = 0;
   std::string err;
   Grid grid() const;
@@ -93,6 +100,23 @@ inline Grid awfl_ctx::grid() const {
   G.prG = d_pref + 2 * cfg.nz;
   G.prF = d_pref + 2 * cfg.nz + ng;
   return G;
+}
+
+// launch constants of an FP64 sweep along dir with time step dt
+inline awfl::SweepK sweep_constants(const awfl_ctx* c, double dt, int dir) {
+  awfl::SweepK K{};
+  double dsw = dir == 0 ? c->dx : dir == 1 ? c->dy : c->dz;
+  K.dt = dt;
+  K.rd = 1.0 / dsw;
+  double dtmult = dt;
+  for (int kt = 0; kt < awfl::KMAX; kt++) {
+    K.ndt[kt] = -1.0 / (dsw * (kt + 1.0));
+    if (kt >= 1) {
+      K.tav[kt] = dtmult / (kt + 1.0);
+      dtmult *= dt;
+    }
+  }
+  return K;
 }
 
 inline int fail(awfl_ctx* c, int code, const std::string& msg) {

@@ -119,6 +119,11 @@ def lib():
         L.awfl_state_from_f32.argtypes = [P, P, P, P]
         L.awfl_sweep_f32.argtypes = [P, P, P, P, P, D, C.c_int]
         L.awfl_time_step_f32.argtypes = [P, P, P, D]
+        L.awfl_sweep_tangent.argtypes = [P, P, P, P, P, D, C.c_int]
+        L.awfl_sweep_adjoint.argtypes = [P, P, P, P, D, C.c_int]
+        L.awfl_time_step_adjoint.argtypes = [P, P, P, D, C.c_int]
+        L.awfl_forward_ckpt.argtypes = [P, P, D, C.c_int, C.c_int, P]
+        L.awfl_adjoint_ckpt.argtypes = [P, P, D, C.c_int, C.c_int, C.c_int, P]
         _lib = L
     return _lib
 
@@ -303,6 +308,51 @@ class Dycore:
             self._chk_f32(t)
         self._check(self.L.awfl_sweep_f32(self.h, self._ptr(s_in), self._ptr(c_in), self._ptr(s_out),
                                           self._ptr(c_out), dt, direction))
+
+    # --- tangent-linear and adjoint (FP64, dt held fixed) ---
+    def sweep_tangent(self, state, dstate, dt, direction):
+        """(sweep(state), J dstate) for one split sweep"""
+        self._chk_state(state)
+        self._chk_state(dstate)
+        out, dout = self.new_state(), self.new_state()
+        self._check(self.L.awfl_sweep_tangent(self.h, self._ptr(state), self._ptr(dstate), self._ptr(out),
+                                              self._ptr(dout), dt, direction))
+        return out, dout
+
+    def sweep_adjoint(self, state, adj_out, dt, direction):
+        """J(state)^T adj_out for one split sweep"""
+        self._chk_state(state)
+        self._chk_state(adj_out)
+        adj_in = self.new_state()
+        self._check(self.L.awfl_sweep_adjoint(self.h, self._ptr(state), self._ptr(adj_out), self._ptr(adj_in),
+                                              dt, direction))
+        return adj_in
+
+    def time_step_adjoint(self, state_before, adj, dt, dim_switch):
+        """adj <- (d step / d state)^T adj in place, for the step taken from
+        state_before with the given dimSwitch"""
+        self._chk_state(state_before)
+        self._chk_state(adj)
+        self._check(self.L.awfl_time_step_adjoint(self.h, self._ptr(state_before), self._ptr(adj), dt,
+                                                  int(dim_switch)))
+
+    def forward_ckpt(self, state, dt, nsteps, every):
+        """nsteps fused steps in place; returns the checkpoint tensor"""
+        self._chk_state(state)
+        nslot = max(1, (nsteps + every - 1) // every)
+        ckpt = self.torch.empty((nslot,) + self.state_shape, dtype=self.torch.float64, device="cuda")
+        self._check(self.L.awfl_forward_ckpt(self.h, self._ptr(state), dt, nsteps, every, self._ptr(ckpt)))
+        return ckpt
+
+    def adjoint_ckpt(self, ckpt, dt, nsteps, every, dim_switch0, adj):
+        """adj: dJ/d(final state) in, dJ/d(initial state) out"""
+        self._chk_state(adj)
+        nslot = max(1, (nsteps + every - 1) // every)
+        if ckpt.dtype != self.torch.float64 or not ckpt.is_cuda or not ckpt.is_contiguous() or \
+                tuple(ckpt.shape) != (nslot,) + self.state_shape:
+            raise ValueError("checkpoints must be the tensor forward_ckpt returned")
+        self._check(self.L.awfl_adjoint_ckpt(self.h, self._ptr(ckpt), dt, nsteps, every, int(dim_switch0),
+                                             self._ptr(adj)))
 
     def state_sums(self, state):
         self._chk_state(state)

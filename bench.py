@@ -32,6 +32,9 @@ BYTES_PER_CELL_SWEEP = 80.0  # read + write 5 doubles
 FLOPS_PER_CELL_SWEEP = 2412.0
 
 NX, NY, NZ = 256, 256, 128
+# FP32 perturbation-form comparison: 512^2 x 256 bubble + noise + band-limited
+# theta' field, 200 steps FP64 vs FP32 from the same initial state
+NX4, NY4, NZ4, STEPS4 = 512, 512, 256, 200
 
 
 def cpu_baseline(nthreads, target_s=12.0):
@@ -50,6 +53,71 @@ def cpu_baseline(nthreads, target_s=12.0):
     return nx * ny * nz * steps / t, steps, t
 
 
+def fp32_leg(steps, warmup):
+    """FP64 and FP32 runs of the 512x512x256 workload, each through its fused
+    C-ABI time step (awfl_time_step / awfl_time_step_f32); throughput of each
+    and the FP32 errors of theta', w and rho' against the FP64 run."""
+    import torch
+    from awflcloud_amd import Dycore, bubble_spec, band_limited_modes
+    from awflcloud_amd.dycore import AWFL_FP32_PERT
+    D = Dycore(NX4, NY4, NZ4, bc=(0, 0, 1), ord=5, ngll=3, precision=AWFL_FP32_PERT)
+    spec = bubble_spec([dict(shape=1, x0=10000.0, y0=10000.0, z0=2000.0, xr=2000.0, yr=2000.0, zr=2000.0,
+                             amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1,
+                       modes=band_limited_modes(seed=4, n=64, kmin=4, kmax=16, rms=0.2))
+    st0 = D.init_state(spec)
+    dt = D.compute_time_step(st0, 0.8)
+
+    def run64(st, n):
+        D.dim_switch = 1
+        for _ in range(n):
+            D.time_step(st, dt)
+
+    def run32(s, c, n):
+        D.dim_switch = 1
+        for _ in range(n):
+            D.time_step_f32(s, c, dt)
+
+    scratch = st0.clone()
+    run64(scratch, warmup)
+    s, c = D.to_f32(st0)
+    run32(s, c, warmup)
+    del scratch
+    st64 = st0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run64(st64, steps)
+    torch.cuda.synchronize()
+    w64 = time.perf_counter() - t0
+    s, c = D.to_f32(st0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run32(s, c, steps)
+    torch.cuda.synchronize()
+    w32 = time.perf_counter() - t0
+    st32 = D.from_f32(s, c)
+    del s, c
+
+    hs, nz = D.hs, D.nz
+    r = torch.tensor(D.background(0)[hs:hs + nz], dtype=torch.float64, device="cuda")[:, None, None]
+    rt = torch.tensor(D.background(3)[hs:hs + nz], dtype=torch.float64, device="cuda")[:, None, None]
+    a, b = D.interior(st32), D.interior(st64)
+    errs = {}
+    pairs = (("theta_K", (rt + a[4]) / (r + a[0]), (rt + b[4]) / (r + b[0])),
+             ("w_m_s", a[3] / (r + a[0]), b[3] / (r + b[0])),
+             ("rho_kg_m3", a[0], b[0]))
+    for name, x, y in pairs:
+        d = x - y
+        errs[name] = {"l2": float(d.pow(2).mean().sqrt()), "linf": float(d.abs().max())}
+    ncell = NX4 * NY4 * NZ4
+    v64, v32 = ncell * steps / w64, ncell * steps / w32
+    return {"workload": "thermal bubble + 1e-3 K noise + band-limited theta' (|k| 4-16, 0.2 K rms, 64 modes) "
+                        "%dx%dx%d, WENO5/ADER3, %d steps" % (NX4, NY4, NZ4, steps),
+            "fp64_cell_updates_per_s": v64, "fp32_cell_updates_per_s": v32, "fp32_over_fp64": v32 / v64,
+            "fp64_ms_per_step": w64 / steps * 1e3, "fp32_ms_per_step": w32 / steps * 1e3,
+            "errors_vs_fp64": errs, "error_norms": "L2 = rms over cells, Linf = max; FP32 minus FP64",
+            "finite": bool(torch.isfinite(st32).all())}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +125,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the 512x512x256 FP32-vs-FP64 leg")
+    ap.add_argument("--fp32-steps", type=int, default=STEPS4)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -135,6 +205,10 @@ def main():
                  "flops_per_cell_sweep": FLOPS_PER_CELL_SWEEP},
         "checks": {"finite": finite, "mass_drift_rel": abs(mass1 - mass0) / max(abs(mass0), 1.0)},
     }
+    if not args.no_fp32:
+        del bufs
+        torch.cuda.empty_cache()
+        out["fp32_pert"] = fp32_leg(args.fp32_steps, 2)
     if not args.no_cpu:
         v1, n1, t1 = cpu_baseline(1, args.cpu_seconds)
         out["cpu_baseline"] = {"value": v1, "unit": "cell-updates/s", "cores": 1, "kind": "port",

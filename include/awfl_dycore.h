@@ -159,6 +159,40 @@ int awfl_sweep_f32(awfl_ctx* ctx, const float* d_s_in, const float* d_c_in, floa
  * awfl_time_step, sharing the context's dimSwitch (T:42-67). */
 int awfl_time_step_f32(awfl_ctx* ctx, float* d_s, float* d_c, double dt);
 
+/* ---- tangent-linear and discrete adjoint (FP64) -----------------------------
+ * Derivatives of the FP64 step with respect to the state, dt held fixed (the
+ * reference caches dt after the first computeTimeStep, S:166).  No
+ * counterpart in the reference.  Available for (ord, ngll) = (3, 2), (5, 3).
+ * Adjoint arrays use the state's padded layout; halo entries of outputs are
+ * zero and halo entries of inputs are ignored. */
+
+/* Tangent-linear of awfl_sweep: d_out = sweep(d_in), d_dout = J(d_in) d_din. */
+int awfl_sweep_tangent(awfl_ctx* ctx, const double* d_in, const double* d_din, double* d_out, double* d_dout,
+                       double dt, int dir);
+
+/* Adjoint of awfl_sweep: d_adj_in = J(d_in)^T d_adj_out (out of place). */
+int awfl_sweep_adjoint(awfl_ctx* ctx, const double* d_in, const double* d_adj_out, double* d_adj_in, double dt,
+                       int dir);
+
+/* Adjoint of one awfl_time_step taken from d_state_before with the context's
+ * dimSwitch equal to dim_switch: d_adj <- (d step / d state)^T d_adj, in
+ * place.  Recomputes the step's split inputs. */
+int awfl_time_step_adjoint(awfl_ctx* ctx, const double* d_state_before, double* d_adj, double dt, int dim_switch);
+
+/* nsteps awfl_time_step calls in place on d_state, copying the state before
+ * steps 0, every, 2*every, ... into consecutive state-sized slots of d_ckpt
+ * (ceil(nsteps/every) slots).  Read awfl_get_dim_switch first: the adjoint
+ * run needs the value it had before the first step. */
+int awfl_forward_ckpt(awfl_ctx* ctx, double* d_state, double dt, int nsteps, int every, double* d_ckpt);
+
+/* Adjoint of the run of awfl_forward_ckpt: d_adj enters as dJ/d(final
+ * state) and leaves as dJ/d(initial state).  Segment by segment, last first,
+ * the split inputs of the segment's steps are recomputed from its checkpoint
+ * (device memory for every*3 states) and the split adjoints applied in
+ * reverse order. */
+int awfl_adjoint_ckpt(awfl_ctx* ctx, const double* d_ckpt, double dt, int nsteps, int every, int dim_switch0,
+                      double* d_adj);
+
 #ifdef __cplusplus
 }
 #endif

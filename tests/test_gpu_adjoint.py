@@ -1,0 +1,237 @@
+"""Tangent-linear and discrete-adjoint sweeps (awfl_sweep_tangent /
+awfl_sweep_adjoint / awfl_time_step_adjoint / awfl_*_ckpt) on the GPU.
+
+Checks, all through the C ABI:
+  * tangent sweep value == awfl_sweep (<= 1e-13 relative);
+  * dot-product identity <y, J x> == <J^T y, x> for each sweep direction and
+    boundary configuration (<= 1e-11 relative);
+  * tangent == fourth-order central differences of awfl_sweep (<= 1e-7);
+  * time-step adjoint == reverse composition of the sweep adjoints;
+  * the checkpointed run's gradient of J = integral of theta'^2 over a
+    4 x 4 x 2 km box centred at (10, 10, 6) km after 60 steps, with respect
+    to the initial theta' field, against central finite differences
+    (eps = 1e-6) along 8 seeded random directions (<= 1e-6 relative), and
+    the dot-product identity of the whole 60-step run.
+"""
+import numpy as np
+import pytest
+
+from helpers import bubble_dicts
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _interior_dot(D, a, b):
+    return float((D.interior(a) * D.interior(b)).sum())
+
+
+def _rand_like_state(D, gen, scale=(1e-3, 1.0, 1.0, 1.0, 0.5)):
+    x = D.new_state()
+    hs = D.hs
+    for l, s in enumerate(scale):
+        x[l, hs:hs + D.nz, hs:hs + D.ny, hs:hs + D.nx] = s * torch.randn(
+            (D.nz, D.ny, D.nx), generator=gen, dtype=torch.float64, device="cuda")
+    return x
+
+
+def _setup(nx, ny, nz, bc=(0, 0, 1), ord=5, ngll=3, u0=4.0, v0=-2.0, seed=7):
+    from awflcloud_amd import Dycore, bubble_spec
+    D = Dycore(nx, ny, nz, bc=bc, ord=ord, ngll=ngll)
+    st = D.init_state(bubble_spec(bubble_dicts(), noise_amp=1e-3, noise_seed=seed, u0=u0, v0=v0,
+                                  y2d=1.0 if ny == 1 else 0.0))
+    dt = D.compute_time_step(st, 0.8)
+    # two steps so that the state carries winds, sound waves and a deformed bubble
+    D.dim_switch = 1
+    D.time_step(st, dt)
+    D.time_step(st, dt)
+    return D, st, dt
+
+
+CASES = [
+    dict(n=(24, 20, 16), bc=(0, 0, 1), ord=5, ngll=3),
+    dict(n=(24, 20, 16), bc=(1, 1, 1), ord=5, ngll=3),
+    dict(n=(24, 20, 16), bc=(0, 0, 0), ord=5, ngll=3),
+    dict(n=(20, 16, 12), bc=(0, 1, 1), ord=3, ngll=2),
+    dict(n=(32, 1, 16), bc=(0, 0, 1), ord=5, ngll=3),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-bc%s-o%d" % ("x".join(map(str, c["n"])),
+                                                                        "".join(map(str, c["bc"])), c["ord"]))
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_sweep_dot_product(case, direction):
+    _need_gpu()
+    nx, ny, nz = case["n"]
+    print("case", case, "dir", direction, flush=True)
+    D, st, dt = _setup(nx, ny, nz, bc=case["bc"], ord=case["ord"], ngll=case["ngll"])
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1234 + direction)
+    x = _rand_like_state(D, gen)
+    y = _rand_like_state(D, gen)
+    out, dout = D.sweep_tangent(st, x, dt, direction)
+    ref = D.new_state()
+    D.sweep(st, ref, dt, direction)
+    vi, ri = D.interior(out), D.interior(ref)
+    verr = float((vi - ri).abs().max() / ri.abs().max())
+    xb = D.sweep_adjoint(st, y, dt, direction)
+    lhs = _interior_dot(D, y, dout)
+    rhs = _interior_dot(D, xb, x)
+    scale = float((D.interior(y) * D.interior(dout)).abs().sum())
+    print("value err %.2e  <y,Jx> %.15e  <J^T y,x> %.15e  rel %.2e" % (verr, lhs, rhs, abs(lhs - rhs) / scale),
+          flush=True)
+    assert verr <= 1e-13
+    assert abs(lhs - rhs) <= 1e-11 * scale
+    # adjoint halo is zero
+    hs = D.hs
+    mask = torch.ones_like(xb, dtype=torch.bool)
+    mask[:, hs:hs + nz, hs:hs + ny, hs:hs + nx] = False
+    assert float(xb[mask].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_sweep_tangent_matches_differences(direction):
+    _need_gpu()
+    D, st, dt = _setup(24, 20, 16, bc=(0, 1, 1))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(99 + direction)
+    x = _rand_like_state(D, gen)
+    _, dout = D.sweep_tangent(st, x, dt, direction)
+    h = 1e-4
+    outs = []
+    for m in (1.0, -1.0, 2.0, -2.0):
+        o = D.new_state()
+        D.sweep(st + m * h * x, o, dt, direction)
+        outs.append(D.interior(o))
+    fd = (8.0 * (outs[0] - outs[1]) - (outs[2] - outs[3])) / (12.0 * h)
+    err = float((fd - D.interior(dout)).abs().max() / D.interior(dout).abs().max())
+    print("dir %d tangent vs fd rel err %.2e" % (direction, err), flush=True)
+    assert err <= 1e-7
+
+
+def test_time_step_adjoint_is_reverse_of_sweeps():
+    _need_gpu()
+    D, st, dt = _setup(24, 20, 16)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    y = _rand_like_state(D, gen)
+    for ds, dirs in ((1, (0, 1, 2)), (0, (2, 1, 0))):
+        # split inputs of one step
+        ins = [st.clone()]
+        for d in dirs[:-1]:
+            o = D.new_state()
+            D.sweep(ins[-1], o, dt, d)
+            ins.append(o)
+        a = y.clone()
+        for d, s_in in zip(reversed(dirs), reversed(ins)):
+            a = D.sweep_adjoint(s_in, a, dt, d)
+        b = y.clone()
+        D.time_step_adjoint(st, b, dt, ds)
+        err = float((D.interior(a) - D.interior(b)).abs().max() / D.interior(a).abs().max())
+        print("dimSwitch %d: step adjoint vs sweeps %.2e" % (ds, err), flush=True)
+        assert err <= 1e-14
+
+
+def test_checkpointed_adjoint_equals_stepwise():
+    """awfl_adjoint_ckpt (segments of 4) == the step-by-step adjoint"""
+    _need_gpu()
+    D, st, dt = _setup(24, 20, 16)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(8)
+    y = _rand_like_state(D, gen)
+    nsteps, every = 10, 4
+    D.dim_switch = 1
+    states = []
+    s = st.clone()
+    for n in range(nsteps):
+        states.append((s.clone(), D.dim_switch))
+        D.time_step(s, dt)
+    a = y.clone()
+    for s_before, ds in reversed(states):
+        D.time_step_adjoint(s_before, a, dt, ds)
+    D.dim_switch = 1
+    s2 = st.clone()
+    ckpt = D.forward_ckpt(s2, dt, nsteps, every)
+    assert torch.equal(s2, s)
+    b = y.clone()
+    D.adjoint_ckpt(ckpt, dt, nsteps, every, 1, b)
+    err = float((D.interior(a) - D.interior(b)).abs().max() / D.interior(a).abs().max())
+    print("checkpointed vs stepwise %.2e" % err, flush=True)
+    assert err <= 1e-14
+
+
+def _c5_problem():
+    from awflcloud_amd import Dycore, bubble_spec
+    from awflcloud_amd import adjoint as A
+    D = Dycore(128, 128, 64)
+    base = D.init_state(bubble_spec(bubble_dicts(), noise_amp=1e-3, noise_seed=1))
+    dt = D.compute_time_step(base, 0.8)
+    mask = A.box_mask(D, (8000.0, 12000.0, 8000.0, 12000.0, 5000.0, 7000.0))
+    return D, base, dt, mask
+
+
+def test_gradient_of_box_objective_against_differences():
+    """128x128x64 bubble, 60 steps, checkpoints every 10: dJ/dtheta'_0 by the
+    adjoint against central differences (eps = 1e-6 K) along 8 directions."""
+    _need_gpu()
+    from awflcloud_amd import adjoint as A
+    D, base, dt, mask = _c5_problem()
+    nsteps, every = 60, 10
+    theta = torch.zeros((D.nz, D.ny, D.nx), dtype=torch.float64, device="cuda", requires_grad=True)
+    J = A.theta_box_objective(D, A.run(A.with_theta(D, base, theta), D, dt, nsteps, every), mask)
+    J.backward()
+    grad = theta.grad.detach()
+    print("J = %.10e, |grad| = %.3e" % (float(J), float(grad.norm())), flush=True)
+
+    def J_of(th):
+        with torch.no_grad():
+            return float(A.theta_box_objective(D, A.run(A.with_theta(D, base, th), D, dt, nsteps, every), mask))
+
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(2024)
+    eps = 1e-6
+    worst = 0.0
+    for k in range(8):
+        d = torch.randn((D.nz, D.ny, D.nx), generator=gen, dtype=torch.float64, device="cuda")
+        fd = (J_of(eps * d) - J_of(-eps * d)) / (2 * eps)
+        ad = float((grad * d).sum())
+        rel = abs(fd - ad) / abs(ad)
+        worst = max(worst, rel)
+        print("direction %d: adjoint %.12e  fd %.12e  rel %.2e" % (k, ad, fd, rel), flush=True)
+    assert worst <= 1e-6
+
+
+def test_whole_run_dot_product():
+    """<y, L x> == <L^T y, x> for the 60-step run, L its tangent-linear map
+    (composition of tangent sweeps) and L^T the checkpointed adjoint."""
+    _need_gpu()
+    D, base, dt, _ = _c5_problem()
+    nsteps, every = 60, 10
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(77)
+    x = _rand_like_state(D, gen)
+    y = _rand_like_state(D, gen)
+    s, dx = base.clone(), x.clone()
+    for n in range(nsteps):
+        dirs = (0, 1, 2) if n % 2 == 0 else (2, 1, 0)
+        for d in dirs:
+            s, dx = D.sweep_tangent(s, dx, dt, d)
+    D.dim_switch = 1
+    s2 = base.clone()
+    ckpt = D.forward_ckpt(s2, dt, nsteps, every)
+    ferr = float((D.interior(s) - D.interior(s2)).abs().max() / D.interior(s2).abs().max())
+    a = y.clone()
+    D.adjoint_ckpt(ckpt, dt, nsteps, every, 1, a)
+    lhs = _interior_dot(D, y, dx)
+    rhs = _interior_dot(D, a, x)
+    scale = float((D.interior(y) * D.interior(dx)).abs().sum())
+    print("tangent-run state vs forward %.2e; <y,Lx> %.15e <L^T y,x> %.15e rel %.2e"
+          % (ferr, lhs, rhs, abs(lhs - rhs) / scale), flush=True)
+    assert ferr <= 1e-10
+    assert abs(lhs - rhs) <= 1e-9 * scale
