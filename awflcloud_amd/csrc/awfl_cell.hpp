@@ -115,7 +115,8 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   });
   // "bridge" polynomial (:123-131)
   sfor<hs + 1>([&](auto i) {
-    sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= (T)W::idl[i] * a[i][ii]; });
+    constexpr T d = (T)W::idl[i];
+    sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= d * a[i][ii]; });
   });
   constexpr T ridl = (T)(1.0 / W::idl[hs + 1]);
   sfor<ORD>([&](auto ii) { a[hs + 1][ii] *= ridl; });
@@ -126,7 +127,8 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   T lo_avg = 0;
   sfor<hs + 1>([&](auto i) { lo_avg += tv[i]; });
   lo_avg *= (T)(1.0 / (hs + 1));
-  tv[hs + 1] = lo_avg + (tv[hs + 1] - lo_avg) * (T)W::sigma;
+  constexpr T sigma = (T)W::sigma;
+  tv[hs + 1] = lo_avg + (tv[hs + 1] - lo_avg) * sigma;
   // weights ~ idl / TV^2, convexify, map, convexify (:153-161).  Each
   // normalisation w_i = (a_i / b_i) / (sum_k a_k / b_k + eps) is formed over
   // the common denominator: a_i prod_{j!=i} b_j / (sum_k a_k prod_{j!=k} b_j
@@ -139,7 +141,8 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
       excl_products<hs + 2>(D, ex, P);
       T N = 0.0;
       sfor<hs + 2>([&](auto i) {
-        w[i] = W::idl[i] * ex[i];
+        constexpr double d = W::idl[i];
+        w[i] = d * ex[i];
         N += w[i];
       });
       T rs = rcp(fma(WENO_EPS, P, N));
@@ -162,20 +165,36 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
       sfor<hs + 2>([&](auto i) { w[i] *= rs; });
     }
   } else {
-    // FP32: products of eps-sized denominators would underflow; one hardware
-    // reciprocal per weight is cheap in FP32
-    sfor<hs + 2>([&](auto i) { w[i] = (T)W::idl[i] * rcp_t(tv[i] * tv[i] + (T)WENO_EPS); });
+    // FP32 (and the tangent-linear scalar): products of the eps-sized TV
+    // denominators would underflow, so the first normalisation takes one
+    // reciprocal per weight; the map's denominators d^2 + w (1 - 2d) are
+    // bounded below by min(d, 1-d)^2 and share one reciprocal
+    sfor<hs + 2>([&](auto i) {
+      constexpr T d = (T)W::idl[i];
+      w[i] = d * rcp_t(tv[i] * tv[i] + (T)WENO_EPS);
+    });
     T sum = 0;
     sfor<hs + 2>([&](auto i) { sum += w[i]; });
     T rs = rcp_t(sum + (T)WENO_EPS);
     sfor<hs + 2>([&](auto i) { w[i] *= rs; });
+    T A[hs + 2], E[hs + 2], ex[hs + 2], P;
     sfor<hs + 2>([&](auto i) {
       constexpr T d = (T)W::idl[i];
-      w[i] = w[i] * (d + d * d - (T)3 * d * w[i] + w[i] * w[i]) * rcp_t(d * d + w[i] * ((T)1 - (T)2 * d));
+      constexpr T d2 = (T)(W::idl[i] + W::idl[i] * W::idl[i]);
+      constexpr T dd = (T)(W::idl[i] * W::idl[i]);
+      constexpr T c3 = (T)(3.0 * W::idl[i]);
+      constexpr T c12 = (T)(1.0 - 2.0 * W::idl[i]);
+      A[i] = w[i] * (d2 - c3 * w[i] + w[i] * w[i]);
+      E[i] = dd + w[i] * c12;
+      (void)d;
     });
+    excl_products<hs + 2>(E, ex, P);
     sum = 0;
-    sfor<hs + 2>([&](auto i) { sum += w[i]; });
-    rs = rcp_t(sum + (T)WENO_EPS);
+    sfor<hs + 2>([&](auto i) {
+      w[i] = A[i] * ex[i];
+      sum += w[i];
+    });
+    rs = rcp_t(sum + (T)WENO_EPS * P);
     sfor<hs + 2>([&](auto i) { w[i] *= rs; });
   }
   // weighted sum (:163-171); candidates have no coefficients above hs

@@ -55,10 +55,15 @@ struct CellBgP {
 };
 
 // Launch constants in FP32
+// Launch constants in FP32.  dn[kt][s][ii] = D[s][ii] * ndt[kt]: the GLL
+// derivative matrix with the transform factor -1/(d (kt+1)) folded in
+// (ngll <= 3 here).
+constexpr int PK_NA = 3, PK_NG = 3;
 struct SweepKF {
   float dt, rd;
   float ndt[KMAX];
   float tav[KMAX];
+  float dn[PK_NA - 1][PK_NG][PK_NG];
 };
 
 // One cell of a DIR sweep in perturbation form: deviation stencils -> deviation
@@ -69,6 +74,7 @@ __device__ __forceinline__ void cell_edges_pert(const float (&st)[5][ORD], const
                                                 float& src_avg) {
   constexpr int NA = NGLL;
   constexpr int NM = 1 + DIR;
+  static_assert(NA <= PK_NA && NGLL <= PK_NG, "FP32 launch constants sized for ngll <= 3");
   float q[5][NA][NGLL];    // level 0: deviations for rho and rho*theta; other levels deviations throughout
   float fr0[NGLL], frt0[NGLL];   // full rho and rho*theta at level 0
   float f[4][NA][NGLL];
@@ -92,12 +98,11 @@ __device__ __forceinline__ void cell_edges_pert(const float (&st)[5][ORD], const
     fr0[ii] = b.r + q[idR][0][ii];
     frt0[ii] = b.rt + q[idT][0][ii];
     rr[ii] = __builtin_amdgcn_rcpf(fr0[ii]);
-    float vel[5];
-    vel[1] = q[1][0][ii] * rr[ii];
-    vel[2] = q[2][0][ii] * rr[ii];
-    vel[3] = q[3][0][ii] * rr[ii];
-    vel[4] = frt0[ii] * rr[ii];
-    sfor<4>([&](auto m) { f[m][0][ii] = fr0[ii] * vel[NM] * vel[m + 1]; });
+    const float un = q[NM][0][ii] * rr[ii];   // r * (q_nm / r) * (q_l / r) = (q_nm / r) * q_l
+    f[0][0][ii] = un * q[1][0][ii];
+    f[1][0][ii] = un * q[2][0][ii];
+    f[2][0][ii] = un * q[3][0][ii];
+    f[3][0][ii] = un * frt0[ii];
     float s = powm1_gamma(q[idT][0][ii] * b.rrt);
     pp[ii] = b.p * s;
     g[0][ii] = (b.p + pp[ii]) * (1.0f / C0F);
@@ -110,13 +115,12 @@ __device__ __forceinline__ void cell_edges_pert(const float (&st)[5][ORD], const
 
   sfor<NA - 1>([&](auto kt) {
     constexpr float rkt1 = 1.0f / (kt + 1.0f);
-    const float ndt = K.ndt[kt];
     sfor<NGLL>([&](auto ii) {
       float d[5];
       sfor<5>([&](auto l) { d[l] = -0.0f; });
       sfor<NGLL>([&](auto s) {
-        constexpr float D = (float)Deriv<NGLL>::m[s][ii];
-        if constexpr (D == 0.0f) return;
+        if constexpr (Deriv<NGLL>::m[s][ii] == 0.0) return;
+        const float D = K.dn[kt][s][ii];
         d[0] += D * q[NM][kt][s];
         sfor<4>([&](auto m) {
           constexpr int l = m + 1;
@@ -131,8 +135,8 @@ __device__ __forceinline__ void cell_edges_pert(const float (&st)[5][ORD], const
         });
       });
       sfor<5>([&](auto l) {
-        if constexpr (DIR == 2 && l == idW) q[l][kt + 1][ii] = d[l] * ndt + src[kt][ii] * rkt1;
-        else q[l][kt + 1][ii] = d[l] * ndt;
+        if constexpr (DIR == 2 && l == idW) q[l][kt + 1][ii] = d[l] + src[kt][ii] * rkt1;
+        else q[l][kt + 1][ii] = d[l];
       });
       if constexpr (DIR == 2) {
         if (bg.wall_top) q[idW][kt + 1][NGLL - 1] = 0.0f;

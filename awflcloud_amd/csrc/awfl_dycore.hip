@@ -55,8 +55,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVE
   extern __shared__ double sm[];
   const int nx = G.nx;
   const int W = nx + 2 * hs;
-  const int j = blockIdx.x % G.ny;
-  const int k = blockIdx.x / G.ny;
+  const unsigned line = tile_index();
+  const int j = line % G.ny;
+  const int k = line / G.ny;
   double* s_st = sm;              // [5][W]   state of the line with boundary cells
   double* s_L = sm + 5 * W;       // [5][nx]  left edge states
   double* s_R = s_L + 5 * nx;     // [5][nx]  right edge states
@@ -153,15 +154,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
                                                   SweepK K, int seglen) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const dim3 tb = tile_coords();
+  const int i = tb.x * blockDim.x + threadIdx.x;
   if (i >= G.nx) return;              // no barrier in this kernel
-  const int o = blockIdx.y;           // y sweep: k; z sweep: j
+  const int o = tb.y;                 // y sweep: k; z sweep: j
   const int n = DIR == 1 ? G.ny : G.nz;
   const int64_t sa = DIR == 1 ? G.sy : G.sz;
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   const int64_t base = DIR == 1 ? (int64_t)(hs + o) * G.sz + (int64_t)hs * G.sy + hs + i
                                 : (int64_t)hs * G.sz + (int64_t)(hs + o) * G.sy + hs + i;
-  const int s0 = blockIdx.z * seglen;
+  const int s0 = tb.z * seglen;
   const int s1 = min(n, s0 + seglen);
 
   auto ld = [&](int l, int a) -> double {
@@ -300,6 +302,27 @@ __global__ void k_apply(double* __restrict__ st, const double* __restrict__ tend
 __global__ void k_eos(const double* __restrict__ rt, double* __restrict__ p, int64_t n) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
     p[t] = C0 * pow_gamma(rt[t]);
+}
+
+// S:1752-1785: diagnostics dens_pert, u, v, w, pot_temp_pert, pressure_pert
+// of the interior, out[6][nz][ny][nx]
+__global__ void k_diag(const double* __restrict__ st, Grid G, double* __restrict__ out) {
+  const int64_t n = (int64_t)G.nx * G.ny * G.nz;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % G.nx);
+    const int j = (int)((t / G.nx) % G.ny);
+    const int k = (int)(t / ((int64_t)G.nx * G.ny));
+    const int64_t c = (int64_t)(G.hs + k) * G.sz + (int64_t)(G.hs + j) * G.sy + G.hs + i;
+    const double rb = G.hyDensCells[G.hs + k], rtb = G.hyDensThetaCells[G.hs + k];
+    const double r = st[idR * G.sl + c] + rb;
+    const double th = (st[idT * G.sl + c] + rtb) / r;
+    out[t] = st[idR * G.sl + c];
+    out[n + t] = st[idU * G.sl + c] / r;
+    out[2 * n + t] = st[idV * G.sl + c] / r;
+    out[3 * n + t] = st[idW * G.sl + c] / r;
+    out[4 * n + t] = th - rtb / rb;
+    out[5 * n + t] = C0 * pow_gamma(r * th) - G.hyPressureCells[G.hs + k];
+  }
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
@@ -659,6 +682,7 @@ int awfl_create(const awfl_config* cfg, awfl_ctx** out) {
   packed.insert(packed.end(), c->hy[4].begin(), c->hy[4].end());
   packed.insert(packed.end(), c->hy[7].begin(), c->hy[7].end());
   packed.insert(packed.end(), c->hy[5].begin(), c->hy[5].end());
+  packed.insert(packed.end(), c->hy[1].begin(), c->hy[1].end());
   (void)nzh; (void)ng;
   e = hipMalloc(&c->d_hy, packed.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(c->d_hy, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -795,6 +819,16 @@ int awfl_eos_pressure(awfl_ctx* c, const double* d_rt, double* d_p, int64_t n) {
   if (n == 0) return AWFL_OK;
   int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
   k_eos<<<blocks, 256, 0, c->stream>>>(d_rt, d_p, n);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
+int awfl_diagnostics(awfl_ctx* c, const double* d_state, double* d_out) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state || !d_out) return fail(c, AWFL_E_ARG, "null argument");
+  int64_t n = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  k_diag<<<blocks, 256, 0, c->stream>>>(d_state, c->grid(), d_out);
   LAUNCH_CHECK(c);
   return AWFL_OK;
 }

@@ -28,6 +28,7 @@ struct Grid {
   const double* hyDensGLL;       // nz*ngll
   const double* hyDensThetaGLL;
   const double* hyPressureGLL;
+  const double* hyPressureCells; // nz+2hs (diagnostics only)
   const PowRef* prT;             // [nz]   x/y: B = hyDensThetaCells(hs+k)
   const PowRef* prRT;            // [nz]   y faces: B = hyDensCells * hyDensThetaCells
   const PowRef* prG;             // [nz*ngll] z: B = hyDensThetaGLL(k, kk)
@@ -42,6 +43,34 @@ bool has_f32_variant(int ord, int ngll);
 }  // namespace awfl_detail
 
 using awfl_detail::Grid;
+
+// ---------------------------------------------------------------------------
+// XCD-aware tile order.  The 8 XCDs of the MI355X receive work-groups
+// round-robin by linear id and each has its own L2.  tile_index() renumbers
+// the grid so that consecutive tile numbers (neighbouring x-blocks of one
+// row, neighbouring rows) run on the same XCD at about the same time and
+// share the cache lines that straddle their boundaries.
+// ---------------------------------------------------------------------------
+constexpr unsigned AWFL_NUM_XCD = 8;
+
+__device__ __forceinline__ unsigned tile_index() {
+  const unsigned n = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned xcd = lin % AWFL_NUM_XCD, local = lin / AWFL_NUM_XCD;
+  const unsigned q = n / AWFL_NUM_XCD, r = n % AWFL_NUM_XCD;
+  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+// tile_index() decoded into the (x, y, z) block coordinates of the grid
+__device__ __forceinline__ dim3 tile_coords() {
+  unsigned t = tile_index();
+  dim3 b;
+  b.x = t % gridDim.x;
+  t /= gridDim.x;
+  b.y = t % gridDim.y;
+  b.z = t / gridDim.y;
+  return b;
+}
 
 // ---------------------------------------------------------------------------
 // context
@@ -95,6 +124,7 @@ inline Grid awfl_ctx::grid() const {
   G.hyDensGLL = d_hy + 2 * nzh_i;
   G.hyDensThetaGLL = d_hy + 2 * nzh_i + ng;
   G.hyPressureGLL = d_hy + 2 * nzh_i + 2 * ng;
+  G.hyPressureCells = d_hy + 2 * nzh_i + 3 * ng;
   G.prT = d_pref;
   G.prRT = d_pref + cfg.nz;
   G.prG = d_pref + 2 * cfg.nz;

@@ -50,12 +50,13 @@ __global__ __launch_bounds__(256) void k_sweep_x_f32(const float* __restrict__ i
   extern __shared__ float smf[];
   const int nx = G.nx;
   const int W = nx + 2 * hs;
-  const int j = blockIdx.x % G.ny;
-  const int k = blockIdx.x / G.ny;
+  const unsigned line = tile_index();
+  const int j = line % G.ny;
+  const int k = line / G.ny;
   float* s_st = smf;
   float* s_L = smf + 5 * W;
   float* s_R = s_L + 5 * nx;
-  float* s_F = smf;
+  float* s_F = s_R + 5 * nx;           // own region: the staged s is still read at the update
   const int64_t row = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs;
   const bool per = G.bc[0] == AWFL_BC_PERIODIC;
   for (int c = threadIdx.x; c < W; c += blockDim.x) {
@@ -73,8 +74,9 @@ __global__ __launch_bounds__(256) void k_sweep_x_f32(const float* __restrict__ i
     }
 #pragma unroll
     for (int l = 0; l < 5; l++) {
-      // the stencils see the represented value s - c
-      float v = in[l * G.sl + row + src] - cin[l * G.sl + row + src];
+      // the stencils see s; the compensation c (below 1/2 ulp of s) enters
+      // only the compensated update
+      float v = in[l * G.sl + row + src];
       s_st[l * W + c] = (zn && l == idU) ? 0.0f : v;
     }
   }
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_f32(const float* __restrict__ i
       float td = -(s_F[l * (nx + 1) + cp] - s_F[l * (nx + 1) + c]) * K.rd;
       if (G.sim2d && l == idV) td = 0.0f;
       const int64_t o = l * G.sl + row + c;
-      kahan(in[o], cin[o], K.dt * td, out[o], cout[o]);
+      kahan(s_st[l * W + c + hs], cin[o], K.dt * td, out[o], cout[o]);
     }
   }
 }
@@ -139,15 +141,16 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
                                                       SweepKF K, int seglen) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const dim3 tb = tile_coords();
+  const int i = tb.x * blockDim.x + threadIdx.x;
   if (i >= G.nx) return;
-  const int o = blockIdx.y;
+  const int o = tb.y;
   const int n = DIR == 1 ? G.ny : G.nz;
   const int64_t sa = DIR == 1 ? G.sy : G.sz;
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   const int64_t base = DIR == 1 ? (int64_t)(hs + o) * G.sz + (int64_t)hs * G.sy + hs + i
                                 : (int64_t)hs * G.sz + (int64_t)(hs + o) * G.sy + hs + i;
-  const int s0 = blockIdx.z * seglen;
+  const int s0 = tb.z * seglen;
   const int s1 = min(n, s0 + seglen);
   auto ld = [&](int l, int a) -> float {
     int src;
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
       src = a;
     }
     const int64_t idx = l * G.sl + base + (int64_t)src * sa;
-    float v = in[idx] - cin[idx];
+    float v = in[idx];                 // s; c enters only the update
     return (zn && l == NM) ? 0.0f : v;
   };
   auto bg_of = [&](int kc, CellBgP<NGLL>& bg) {
@@ -178,14 +181,14 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     }
   };
   auto face_bg = [&](int f) -> FaceBgP { return DIR == 1 ? G.fbL[o] : G.fbF[f]; };
-  auto emit = [&](int c, const float (&F0)[5], const float (&F1)[5], float src) {
+  auto emit = [&](int c, const float (&F0)[5], const float (&F1)[5], float src, const float (&sc)[5]) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       float td = -(F1[l] - F0[l]) * K.rd;
       if (DIR == 2 && l == idW) td = src + td;
       if (DIR == 2 && G.sim2d && l == idV) td = 0.0f;
       const int64_t idx = l * G.sl + base + (int64_t)c * sa;
-      kahan(in[idx], cin[idx], K.dt * td, out[idx], cout[idx]);
+      kahan(sc[l], cin[idx], K.dt * td, out[idx], cout[idx]);
     }
   };
 
@@ -199,7 +202,10 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     cell_edges_pert<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
   }
   for (int v = s0; v < s1; v++) {
+    // the window is centred on cell v-1, the cell updated in this iteration
+    float prevS[5];
     sfor<5>([&](auto l) {
+      prevS[l] = w[l][hs];
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
       w[l][ORD - 1] = ld(l, v + hs);
     });
@@ -211,7 +217,7 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     const FaceBgP fb = face_bg(v);
     if (!per && v == 0) riemann_pert<DIR>(eL, eL, fb, F);
     else riemann_pert<DIR>(prevR, eL, fb, F);
-    if (v > s0) emit(v - 1, prevF, F, prevSrc);
+    if (v > s0) emit(v - 1, prevF, F, prevSrc, prevS);
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       prevF[l] = F[l];
@@ -219,7 +225,8 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     }
     prevSrc = src;
   }
-  float F[5];
+  float F[5], lastS[5];
+  sfor<5>([&](auto l) { lastS[l] = w[l][hs]; });   // cell s1-1
   const FaceBgP fb = face_bg(s1);
   if (!per && s1 == n) {
     riemann_pert<DIR>(prevR, prevR, fb, F);
@@ -234,7 +241,7 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     cell_edges_pert<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
     riemann_pert<DIR>(prevR, eL, fb, F);
   }
-  emit(s1 - 1, prevF, F, prevSrc);
+  emit(s1 - 1, prevF, F, prevSrc, lastS);
 }
 
 __global__ void k_to_f32(const double* __restrict__ d, float* __restrict__ s, float* __restrict__ c, int64_t n) {
@@ -353,6 +360,13 @@ SweepKF sweep_constants_f(const awfl_ctx* c, double dt, int dir) {
       dtmult *= dt;
     }
   }
+  const int ng = c->cfg.ngll;
+  for (int kt = 0; kt + 1 < ng && kt < PK_NA - 1; kt++)
+    for (int s = 0; s < ng; s++)
+      for (int ii = 0; ii < ng; ii++) {
+        double D = ng == 2 ? awfl_deriv_2[s][ii] : awfl_deriv_3[s][ii];
+        K.dn[kt][s][ii] = (float)(D * (-1.0 / (dsw * (kt + 1.0))));
+      }
   return K;
 }
 
@@ -363,7 +377,7 @@ int launch_sweep_f32(awfl_ctx* c, const float* s, const float* cs, float* so, fl
   if (dir == 0) {
     int nx = c->cfg.nx;
     int threads = std::min(256, (nx + 63) / 64 * 64);
-    size_t lds = sizeof(float) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx);
+    size_t lds = sizeof(float) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx + 5 * (nx + 1));
     if (lds > 64 * 1024)
       HIP_TRY(c, hipFuncSetAttribute((const void*)v->sweep_x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     v->sweep_x<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(s, cs, so, co, F, K);

@@ -118,6 +118,86 @@ def fp32_leg(steps, warmup):
             "finite": bool(torch.isfinite(st32).all())}
 
 
+def weno9_leg(steps=50, warmup=2):
+    """WENO9/ADER5 FP64, 192x192x96, two opposite seeded bubbles: fused steps
+    through awfl_time_step, with the per-sweep mean kernel time."""
+    import torch
+    from awflcloud_amd import Dycore
+    from awflcloud_amd.workloads import two_bubbles_spec
+    nx, ny, nz = 192, 192, 96
+    D = Dycore(nx, ny, nz, bc=(0, 0, 1), ord=9, ngll=5)
+    st = D.init_state(two_bubbles_spec(seed=3))
+    dt = D.compute_time_step(st, 0.8)
+    D.dim_switch = 1
+    for _ in range(warmup):
+        D.time_step(st, dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        D.time_step(st, dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = torch.empty_like(st)
+    kms = {}
+    for d in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            D.sweep(st, out, dt, d)
+        e1.record()
+        torch.cuda.synchronize()
+        kms["xyz"[d]] = e0.elapsed_time(e1) / 5
+    return {"workload": "two opposite bubbles (+2 K r 1.5 km, -1.5 K r 2.5 km, seed 3) %dx%dx%d, WENO9/ADER5, "
+                        "FP64, %d steps" % (nx, ny, nz, steps),
+            "cell_updates_per_s": nx * ny * nz * steps / wall, "ms_per_step": wall / steps * 1e3,
+            "sweep_ms": kms, "finite": bool(torch.isfinite(st).all())}
+
+
+def adjoint_leg(reps=3):
+    """Discrete adjoint on the 128x128x64 bubble: 60 FP64 steps with a
+    checkpoint every 10 (awfl_forward_ckpt) against the adjoint run
+    (awfl_adjoint_ckpt: recompute each segment, then 180 split-sweep
+    adjoints); and one split-sweep adjoint against one forward sweep."""
+    import torch
+    from awflcloud_amd import Dycore, bubble_spec
+    nx, ny, nz, nsteps, every = 128, 128, 64, 60, 10
+    D = Dycore(nx, ny, nz, bc=(0, 0, 1), ord=5, ngll=3)
+    base = D.init_state(bubble_spec([dict(shape=1, x0=10000.0, y0=10000.0, z0=2000.0, xr=2000.0, yr=2000.0,
+                                          zr=2000.0, amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1))
+    dt = D.compute_time_step(base, 0.8)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    fwd, bwd = [], []
+    for r in range(reps + 1):
+        st = base.clone()
+        D.dim_switch = 1
+        box = {}
+        fwd.append(timed(lambda: box.update(ckpt=D.forward_ckpt(st, dt, nsteps, every))))
+        adj = torch.randn_like(st)
+        bwd.append(timed(lambda: D.adjoint_ckpt(box["ckpt"], dt, nsteps, every, 1, adj)))
+    fwd, bwd = min(fwd[1:]), min(bwd[1:])
+    sweep = {}
+    out = torch.empty_like(base)
+    y = torch.randn_like(base)
+    for d in range(3):
+        D.sweep(base, out, dt, d)
+        D.sweep_adjoint(base, y, dt, d)
+        tf = timed(lambda: [D.sweep(base, out, dt, d) for _ in range(10)]) / 10
+        ta = timed(lambda: [D.sweep_adjoint(base, y, dt, d) for _ in range(10)]) / 10
+        sweep["xyz"[d]] = {"forward_ms": tf * 1e3, "adjoint_ms": ta * 1e3, "adjoint_over_forward": ta / tf}
+    return {"workload": "thermal bubble 128x128x64, WENO5/ADER3, FP64, %d steps, checkpoint every %d"
+                        % (nsteps, every),
+            "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3, "backward_over_forward": bwd / fwd,
+            "note": "backward = segment recomputation + split-sweep adjoints",
+            "sweep": sweep}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +207,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-fp32", action="store_true", help="skip the 512x512x256 FP32-vs-FP64 leg")
     ap.add_argument("--fp32-steps", type=int, default=STEPS4)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing leg")
+    ap.add_argument("--no-weno9", action="store_true", help="skip the WENO9/ADER5 leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -209,6 +291,12 @@ def main():
         del bufs
         torch.cuda.empty_cache()
         out["fp32_pert"] = fp32_leg(args.fp32_steps, 2)
+    if not args.no_adjoint:
+        torch.cuda.empty_cache()
+        out["adjoint"] = adjoint_leg()
+    if not args.no_weno9:
+        torch.cuda.empty_cache()
+        out["weno9"] = weno9_leg()
     if not args.no_cpu:
         v1, n1, t1 = cpu_baseline(1, args.cpu_seconds)
         out["cpu_baseline"] = {"value": v1, "unit": "cell-updates/s", "cores": 1, "kind": "port",

@@ -21,6 +21,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -34,6 +35,7 @@
 #include <vector>
 
 #include "../awfl_dycore.h"
+#include "netcdf_classic.hpp"
 
 namespace awfl {
 
@@ -276,32 +278,52 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
     return host_tend_[(((size_t)loc.l * nz + loc.k) * ny + loc.j) * nx + loc.i];
   }
 
-  // S:1705-1791 writes netCDF (not available here); this writes the same six
-  // diagnostics as raw float64 [nz][ny][nx] records appended to outFile.bin
+  // S:1704-1791: appends the diagnostics at elapsed time etime to the netCDF
+  // file outFile (created when etime == 0): coordinates x, y, z, background
+  // hyDens/hyPressure/hyTheta, the unlimited t, and dens_pert, u, v, w,
+  // pot_temp_pert, pressure_pert on (t, z, y, x), computed on the device
+  // (awfl_diagnostics); classic 64-bit-offset format (netcdf_classic.hpp)
   void output(StateArr const& state, TracerArr const&, real etime) {
-    std::vector<double> h = state.createHostCopy();
-    std::vector<double> rb(nz + 2 * hs), rtb(nz + 2 * hs), pb(nz + 2 * hs);
-    check(awfl_get_background(ctx_, 0, rb.data(), rb.size()));
-    check(awfl_get_background(ctx_, 1, pb.data(), pb.size()));
-    check(awfl_get_background(ctx_, 3, rtb.data(), rtb.size()));
-    const double C0 = 27.5629410929725921310572974482, GAMMA = 1.40027894002789400278940027894;
-    std::ofstream f(outFile + ".bin", etime == 0 ? std::ios::binary : std::ios::binary | std::ios::app);
-    f.write(reinterpret_cast<const char*>(&etime), sizeof etime);
-    auto at = [&](int l, int k, int j, int i) {
-      return h[(((size_t)l * (nz + 2 * hs) + hs + k) * (ny + 2 * hs) + hs + j) * (nx + 2 * hs) + hs + i];
-    };
-    for (int var = 0; var < 6; var++)
-      for (int k = 0; k < nz; k++)
-        for (int j = 0; j < ny; j++)
-          for (int i = 0; i < nx; i++) {
-            double r = at(idR, k, j, i) + rb[hs + k], v;
-            double t = (at(idT, k, j, i) + rtb[hs + k]) / r;
-            if (var == 0) v = at(idR, k, j, i);                                  // dens_pert
-            else if (var <= 3) v = at(var, k, j, i) / r;                         // u, v, w
-            else if (var == 4) v = t - rtb[hs + k] / rb[hs + k];                 // pot_temp_pert
-            else v = C0 * std::pow(r * t, GAMMA) - pb[hs + k];                    // pressure_pert
-            f.write(reinterpret_cast<const char*>(&v), sizeof v);
-          }
+    static const char* names[6] = {"dens_pert", "u", "v", "w", "pot_temp_pert", "pressure_pert"};
+    const size_t n = (size_t)nx * ny * nz;
+    double* d_out = nullptr;
+    if (hipMalloc(&d_out, 6 * n * sizeof(double)) != hipSuccess) endrun("output: device allocation failed");
+    int rc = awfl_diagnostics(ctx_, state.data(), d_out);
+    std::vector<double> h(6 * n);
+    hipError_t e = rc == AWFL_OK ? hipMemcpy(h.data(), d_out, 6 * n * sizeof(double), hipMemcpyDeviceToHost)
+                                 : hipSuccess;
+    (void)hipFree(d_out);
+    check(rc);
+    if (e != hipSuccess) endrun("output: copy to host failed");
+    nc::ClassicFile f;
+    int64_t rec = 0;
+    if (etime == 0) {
+      std::vector<nc::Dim> dims = {{"x", nx}, {"y", ny}, {"z", nz}, {"t", 0}};
+      std::vector<nc::Var> vars = {{"x", {0}}, {"y", {1}}, {"z", {2}}, {"hyDens", {2}}, {"hyPressure", {2}},
+                                   {"hyTheta", {2}}, {"t", {3}}};
+      for (const char* v : names) vars.push_back({v, {3, 2, 1, 0}});
+      f.create(outFile, dims, vars);
+      std::vector<double> c(std::max(nx, std::max(ny, nz)));
+      for (int i = 0; i < nx; i++) c[i] = (i + 0.5) * dx;
+      f.write_fixed("x", c.data());
+      for (int j = 0; j < ny; j++) c[j] = (j + 0.5) * dy;
+      f.write_fixed("y", c.data());
+      for (int k = 0; k < nz; k++) c[k] = (k + 0.5) * dz;
+      f.write_fixed("z", c.data());
+      const int which[3] = {0, 1, 2};
+      const char* bnames[3] = {"hyDens", "hyPressure", "hyTheta"};
+      std::vector<double> b(nz + 2 * hs);
+      for (int w = 0; w < 3; w++) {
+        check(awfl_get_background(ctx_, which[w], b.data(), (int64_t)b.size()));
+        f.write_fixed(bnames[w], b.data() + hs);
+      }
+    } else {
+      f.open(outFile);
+      rec = f.num_records();
+    }
+    double t = etime;
+    f.write_record("t", rec, &t);
+    for (int v = 0; v < 6; v++) f.write_record(names[v], rec, h.data() + v * n);
   }
 
   void finalize(StateArr const&, TracerArr const&) {}

@@ -129,6 +129,10 @@ int awfl_get_background(const awfl_ctx* ctx, int which, double* h_out, int64_t c
  * the device, with the kernels' own power routine: d_p[i] for d_rt[i] > 0. */
 int awfl_eos_pressure(awfl_ctx* ctx, const double* d_rt, double* d_p, int64_t n);
 
+/* The six output fields of Spatial::output (S:1752-1785) on the device, into
+ * d_out[6][nz][ny][nx]: dens_pert, u, v, w, pot_temp_pert, pressure_pert. */
+int awfl_diagnostics(awfl_ctx* ctx, const double* d_state, double* d_out);
+
 /* Global sums over the interior of each state variable, in FP64 on the device
  * (conservation diagnostics).  h_out[5]. Synchronises. */
 int awfl_state_sums(awfl_ctx* ctx, const double* d_state, double* h_out);

@@ -40,5 +40,18 @@ def test_driver_runs(tmp_path, fused):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "Walltime" in r.stdout and "Steps: 10" in r.stdout
-    out = tmp_path / "thermal_64x64x32.bin"
-    assert out.exists() and out.stat().st_size == 8 * (1 + 6 * 64 * 64 * 32)
+    # netCDF output (S:1704-1791): records at t = 0 and every outFreq = 3.6 s
+    from scipy.io import netcdf_file
+    import numpy as np
+    with netcdf_file(str(tmp_path / "thermal_64x64x32.nc"), "r", mmap=False) as f:
+        assert (f.dimensions["x"], f.dimensions["y"], f.dimensions["z"]) == (64, 64, 32)
+        t = f.variables["t"][:].copy()
+        assert len(t) == 3 and t[0] == 0.0 and 3.6 <= t[1] < 3.6 + 0.8 and 7.2 <= t[2] < 7.2 + 0.8
+        for name in ("dens_pert", "u", "v", "w", "pot_temp_pert", "pressure_pert"):
+            assert f.variables[name].shape == (3, 32, 64, 64)
+        assert np.allclose(f.variables["x"][:], (np.arange(64) + 0.5) * 312.5)
+        assert np.allclose(f.variables["z"][:], (np.arange(32) + 0.5) * 312.5)
+        assert np.all(np.diff(f.variables["hyDens"][:]) < 0)
+        th0 = f.variables["pot_temp_pert"][0]
+        assert 1.5 < th0.max() <= 2.0 + 2e-3 and abs(th0.min()) <= 2e-3
+        assert np.abs(f.variables["w"][0]).max() == 0.0 and np.abs(f.variables["w"][2]).max() > 0.0

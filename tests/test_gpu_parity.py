@@ -236,6 +236,32 @@ def test_thermal_64x64x32_100_steps():
     assert abs(m1[0] - m0[0]) <= 1e-12 * float(D.interior(st)[0].abs().sum())
 
 
+def test_weno9_two_bubbles_steps():
+    """WENO9/ADER5 on two opposite seeded bubbles (the high-order workload at
+    a size the oracle runs in seconds): 20 steps against the oracle."""
+    _need_gpu()
+    from awflcloud_amd.workloads import two_bubbles_spec
+    D, O = _pair(48, 40, 32, ord=9, ngll=5, nthreads=8)
+    st = D.init_state(two_bubbles_spec(seed=3))
+    host = st.cpu().numpy().copy()
+    dt = D.compute_time_step(st, 0.8)
+    O.set_dt_cache(dt)
+    rbar, rtbar = _bg(O)
+    O.dim_switch = 1
+    D.dim_switch = 1
+    for s in range(20):
+        D.time_step(st, dt)
+        O.time_step(host, dt)
+        if s == 0:
+            g = D.interior(st).cpu().numpy()
+            f1 = full_rel_err(g, O.interior(host), rbar, rtbar)
+    g = D.interior(st).cpu().numpy()
+    e20 = per_var_rel_err(g, O.interior(host))
+    print("ord 9 step 1 full", ["%.2e" % v for v in f1], "step 20 pert", ["%.2e" % v for v in e20])
+    assert max(f1[0], f1[4]) <= 1e-12
+    assert max(e20) <= 1e-10
+
+
 def test_mass_conservation_256x256x128():
     _need_gpu()
     from awflcloud_amd import bubble_spec
