@@ -4,14 +4,19 @@
 // awfl_forward_ckpt / awfl_adjoint_ckpt in include/awfl_dycore.h).
 //
 // One split sweep is out = in + dt * T(in) (awfl_sweep).  Its adjoint
-// adj_in = adj_out + J^T (dt adj_out) runs as four kernels, one thread per
-// cell, over per-cell scratch in HBM:
-//   k_edges      cell forward: time-averaged edge states and z source   [11/cell]
+// adj_in = adj_out + J^T (dt adj_out) runs as five kernels over per-cell
+// scratch in HBM:
+//   k_edges      cell forward, one thread per cell: WENO values at the GLL
+//                points [5 ngll/cell], then time-averaged edge states and
+//                z source                                                [11/cell]
 //   k_face_adj   face adjoints: flux adjoint dt/d (adj_out[right] -
 //                adj_out[left]) pulled back through the Riemann flux to the
 //                two edge states the face used                           [10/cell]
-//   k_cell_adj   cell reverse: recompute the cell's transform levels and
-//                pull the edge and source adjoints back to the stencil   [25/cell]
+//   k_dt_adj     transform reverse, one thread per cell: recompute the
+//                transform levels from the GLL values and pull the edge and
+//                source adjoints back to them                     [5 ngll/cell]
+//   k_weno_adj   WENO reverse, one thread per (cell, variable): GLL-value
+//                adjoints back to the stencil                      [5 ord/cell]
 //   k_gather     adj_in[c] = adj_out[c] + sum of the stencil adjoints of
 //                every cell whose stencil holds c (periodic wrap, or the
 //                wall clamp with its zeroed normal momentum)
@@ -146,10 +151,12 @@ __device__ __forceinline__ void load_stencil(const double* __restrict__ st, cons
   });
 }
 
-// cell forward -> edge record (value; and tangent for Dual)
+// cell forward -> edge record (value; and tangent for Dual); the double
+// variant also keeps the WENO values at the GLL points, Q0[l*NGLL+ii][cell]
 template <int ORD, int NGLL, int DIR, class T>
 __global__ __launch_bounds__(64) void k_edges(const double* __restrict__ st, const double* __restrict__ dst, Grid G,
-                                              SweepK K, double* __restrict__ E, double* __restrict__ Ed) {
+                                              SweepK K, double* __restrict__ E, double* __restrict__ Ed,
+                                              double* __restrict__ Q0) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t nc = ncell(G);
   if (t >= nc) return;
@@ -159,8 +166,11 @@ __global__ __launch_bounds__(64) void k_edges(const double* __restrict__ st, con
   else load_stencil<ORD, DIR>(st, dst, G, p, w);
   const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, p.k);
   adj::Tape<NGLL, T> tp;
-  T eL[5], eR[5], sa;
-  adj::cell_fwd<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, tp, eL, eR, sa);
+  T q0[5][NGLL], eL[5], eR[5], sa;
+  adj::cell_recon<ORD, NGLL>(w, G.weno_s, G.weno_w, q0);
+  if constexpr (std::is_same<T, double>::value)
+    sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { Q0[(l * NGLL + ii) * nc + t] = q0[l][ii]; }); });
+  adj::cell_dt<NGLL, DIR>(q0, bg, K, tp, eL, eR, sa);
   sfor<5>([&](auto l) {
     E[l * nc + t] = adj::val(eL[l]);
     E[(5 + l) * nc + t] = adj::val(eR[l]);
@@ -250,29 +260,55 @@ __global__ __launch_bounds__(64) void k_face_adj(const double* __restrict__ E, c
   if (!per && a == n - 1) do_face(n);
 }
 
-// Cell reverse: edge and source adjoints -> stencil adjoints STB[l*ORD+s][cell]
-template <int ORD, int NGLL, int DIR>
-__global__ __launch_bounds__(64) void k_cell_adj(const double* __restrict__ st, const double* __restrict__ EB,
-                                                 const double* __restrict__ adj_out, Grid G, SweepK K,
-                                                 double* __restrict__ STB) {
+// Transform reverse: edge and source adjoints -> GLL-value adjoints
+// Q0B[l*NGLL+ii][cell]
+template <int NGLL, int DIR>
+__global__ __launch_bounds__(64) void k_dt_adj(const double* __restrict__ Q0, const double* __restrict__ EB,
+                                               const double* __restrict__ adj_out, Grid G, SweepK K,
+                                               double* __restrict__ Q0B) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t nc = ncell(G);
   if (t >= nc) return;
   const Pos p = pos_of(G, t);
-  double w[5][ORD];
-  load_stencil<ORD, DIR>(st, G, p, w);
-  const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, p.k);
-  double eLb[5], eRb[5];
+  double q0[5][NGLL], q0b[5][NGLL], eLb[5], eRb[5];
   sfor<5>([&](auto l) {
+    sfor<NGLL>([&](auto ii) { q0[l][ii] = Q0[(l * NGLL + ii) * nc + t]; });
     eLb[l] = EB[l * nc + t];
     eRb[l] = EB[(5 + l) * nc + t];
   });
+  const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, p.k);
   // z: the source enters rho*w as out += dt * src
   const double srcb = DIR == 2 ? K.dt * adj_out[sidx(G, idW, p)] : 0.0;
-  double stb[5][ORD];
-  sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { stb[l][s] = 0.0; }); });
-  adj::cell_adj<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eLb, eRb, srcb, stb);
-  sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { STB[(l * ORD + s) * nc + t] = stb[l][s]; }); });
+  adj::cell_dt_adj<NGLL, DIR>(q0, bg, K, eLb, eRb, srcb, q0b);
+  sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { Q0B[(l * NGLL + ii) * nc + t] = q0b[l][ii]; }); });
+}
+
+// WENO reverse, one thread per (variable, cell): GLL-value adjoints ->
+// stencil adjoints STB[l*ORD+s][cell]
+template <int ORD, int NGLL, int DIR>
+__global__ __launch_bounds__(256) void k_weno_adj(const double* __restrict__ st, const double* __restrict__ Q0B,
+                                                  Grid G, double* __restrict__ STB) {
+  constexpr int hs = (ORD - 1) / 2;
+  constexpr int NM = 1 + DIR;
+  const int64_t nc = ncell(G);
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= 5 * nc) return;
+  const int l = (int)(g / nc);
+  const int64_t t = g - (int64_t)l * nc;
+  const Pos p = pos_of(G, t);
+  const int n = axis_len<DIR>(G);
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const int a0 = along<DIR>(p);
+  double u[ORD], gb[NGLL], ub[ORD];
+  sfor<ORD>([&](auto s) {
+    bool zn;
+    Pos q = moved<DIR>(p, map_axis(a0 - hs + s, n, per, zn));
+    u[s] = (zn && l == NM) ? 0.0 : st[sidx(G, l, q)];
+    ub[s] = 0.0;
+  });
+  sfor<NGLL>([&](auto ii) { gb[ii] = Q0B[(l * NGLL + ii) * nc + t]; });
+  adj::reconstruct_adj<ORD, NGLL>(u, (l == idR || l == idT) ? G.weno_s : G.weno_w, gb, ub);
+  sfor<ORD>([&](auto s) { STB[(l * ORD + s) * nc + t] = ub[s]; });
 }
 
 // adj_in over the padded layout: zero in the halo, in the interior the
@@ -398,10 +434,11 @@ __global__ __launch_bounds__(256) void k_tlm_update(const double* __restrict__ i
 
 struct VariantA {
   int ord, ngll;
-  void (*edges[3])(const double*, const double*, Grid, SweepK, double*, double*);
-  void (*edges_d[3])(const double*, const double*, Grid, SweepK, double*, double*);
+  void (*edges[3])(const double*, const double*, Grid, SweepK, double*, double*, double*);
+  void (*edges_d[3])(const double*, const double*, Grid, SweepK, double*, double*, double*);
   void (*face[3])(const double*, const double*, Grid, SweepK, double*);
-  void (*cell[3])(const double*, const double*, const double*, Grid, SweepK, double*);
+  void (*dt[3])(const double*, const double*, const double*, Grid, SweepK, double*);
+  void (*weno[3])(const double*, const double*, Grid, double*);
   void (*gather[3])(const double*, const double*, Grid, double*);
   void (*tlm[3])(const double*, const double*, const double*, const double*, Grid, SweepK, double*, double*);
 };
@@ -411,7 +448,8 @@ struct VariantA {
     O, N, {k_edges<O, N, 0, double>, k_edges<O, N, 1, double>, k_edges<O, N, 2, double>},           \
         {k_edges<O, N, 0, Dual>, k_edges<O, N, 1, Dual>, k_edges<O, N, 2, Dual>},                   \
         {k_face_adj<N, 0>, k_face_adj<N, 1>, k_face_adj<N, 2>},                                     \
-        {k_cell_adj<O, N, 0>, k_cell_adj<O, N, 1>, k_cell_adj<O, N, 2>},                            \
+        {k_dt_adj<N, 0>, k_dt_adj<N, 1>, k_dt_adj<N, 2>},                                           \
+        {k_weno_adj<O, N, 0>, k_weno_adj<O, N, 1>, k_weno_adj<O, N, 2>},                            \
         {k_gather<O, 0>, k_gather<O, 1>, k_gather<O, 2>},                                           \
         {k_tlm_update<N, 0>, k_tlm_update<N, 1>, k_tlm_update<N, 2>},                               \
   }
@@ -428,7 +466,7 @@ int ensure_scratch(awfl_ctx* c) {
   if (!find_variant_a(c->cfg.ord, c->cfg.ngll))
     return fail(c, AWFL_E_UNSUPPORTED, "adjoint: (ord, ngll) not compiled in");
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
-  const int per_cell = NE + NEB + 5 * c->cfg.ord;
+  const int per_cell = NE + NEB + 5 * c->cfg.ord + 10 * c->cfg.ngll;
   if (!c->a_scr) HIP_TRY(c, hipMalloc(&c->a_scr, sizeof(double) * nc * per_cell));
   if (!c->a_tmp) HIP_TRY(c, hipMalloc(&c->a_tmp, sizeof(double) * c->state_elems()));
   return AWFL_OK;
@@ -449,14 +487,19 @@ int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* 
   const Grid G = c->grid();
   const SweepK K = sweep_constants(c, dt, dir);
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  const int ng = c->cfg.ngll;
   double* E = c->a_scr;
   double* EB = E + NE * nc;
   double* STB = EB + NEB * nc;
-  v->edges[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(in, nullptr, G, K, E, nullptr);
+  double* Q0 = STB + (int64_t)5 * c->cfg.ord * nc;
+  double* Q0B = Q0 + (int64_t)5 * ng * nc;
+  v->edges[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(in, nullptr, G, K, E, nullptr, Q0);
   LAUNCH_CHECK(c);
   v->face[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(E, adj_out, G, K, EB);
   LAUNCH_CHECK(c);
-  v->cell[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(in, EB, adj_out, G, K, STB);
+  v->dt[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(Q0, EB, adj_out, G, K, Q0B);
+  LAUNCH_CHECK(c);
+  v->weno[dir]<<<blocks_for(5 * nc, 256), 256, 0, c->stream>>>(in, Q0B, G, STB);
   LAUNCH_CHECK(c);
   v->gather[dir]<<<blocks_for(G.sl, 256), 256, 0, c->stream>>>(STB, adj_out, G, adj_in);
   LAUNCH_CHECK(c);
@@ -550,7 +593,7 @@ int awfl_sweep_tangent(awfl_ctx* c, const double* d_in, const double* d_din, dou
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   double* E = c->a_scr;
   double* Ed = E + NE * nc;            // fits: NE <= NEB + 5*ord
-  v->edges_d[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(d_in, d_din, G, K, E, Ed);
+  v->edges_d[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(d_in, d_din, G, K, E, Ed, nullptr);
   LAUNCH_CHECK(c);
   v->tlm[dir]<<<blocks_for(G.sl, 256), 256, 0, c->stream>>>(d_in, d_din, E, Ed, G, K, d_out, d_dout);
   LAUNCH_CHECK(c);

@@ -76,22 +76,26 @@ struct Tape {
   T rr[NGLL], rrt[NGLL];   // 1/r, 1/(r t) at level 0
 };
 
-// One cell of a DIR sweep, generic scalar: the statements of cell_edges
-// (awfl_cell.hpp) with pow/rcp/sqrt through pow_g/rcp_g.
-template <int ORD, int NGLL, int DIR, class T>
-AWFL_HD void cell_fwd(const T (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s, bool weno_w, const SweepK& K,
-                      Tape<NGLL, T>& tp, T (&eL)[5], T (&eR)[5], T& src_avg) {
+// WENO stage of one cell: the 5 reconstructions to the GLL points
+// (deviations from the background)
+template <int ORD, int NGLL, class T>
+AWFL_HD void cell_recon(const T (&st)[5][ORD], bool weno_s, bool weno_w, T (&q0)[5][NGLL]) {
+  sfor<5>([&](auto l) { reconstruct<ORD, NGLL>(st[l], (l == idR || l == idT) ? weno_s : weno_w, q0[l]); });
+}
+
+// Transform stage of one cell, generic scalar: from the reconstructed GLL
+// values, the statements of cell_edges (awfl_cell.hpp) with pow/rcp/sqrt
+// through pow_g/rcp_g.
+template <int NGLL, int DIR, class T>
+AWFL_HD void cell_dt(const T (&q0)[5][NGLL], const CellBg<NGLL>& bg, const SweepK& K, Tape<NGLL, T>& tp,
+                     T (&eL)[5], T (&eR)[5], T& src_avg) {
   constexpr int NA = NGLL;
   constexpr int NM = 1 + DIR;
   auto& q = tp.q;
   auto& f = tp.f;
   auto& g = tp.g;
   T src[NA][NGLL];
-  sfor<5>([&](auto l) {
-    T gv[NGLL];
-    reconstruct<ORD, NGLL>(st[l], (l == idR || l == idT) ? weno_s : weno_w, gv);
-    sfor<NGLL>([&](auto ii) { q[l][0][ii] = gv[ii]; });
-  });
+  sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { q[l][0][ii] = q0[l][ii]; }); });
   sfor<NGLL>([&](auto ii) {
     if constexpr (DIR == 2) {
       q[idR][0][ii] += T(bg.rG[ii]);
@@ -197,6 +201,15 @@ AWFL_HD void cell_fwd(const T (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s
   } else {
     src_avg = T(0.0);
   }
+}
+
+// One cell of a DIR sweep, generic scalar: WENO stage then transform stage
+template <int ORD, int NGLL, int DIR, class T>
+AWFL_HD void cell_fwd(const T (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s, bool weno_w, const SweepK& K,
+                      Tape<NGLL, T>& tp, T (&eL)[5], T (&eR)[5], T& src_avg) {
+  T q0[5][NGLL];
+  cell_recon<ORD, NGLL>(st, weno_s, weno_w, q0);
+  cell_dt<NGLL, DIR>(q0, bg, K, tp, eL, eR, src_avg);
 }
 
 // Upwind characteristic flux, generic scalar (statements of riemann in
@@ -536,16 +549,16 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   });
 }
 
-// stb += (d (eL, eR, src_avg) / d st)^T (eLb, eRb, srcb) for one cell
-template <int ORD, int NGLL, int DIR>
-AWFL_HD void cell_adj(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s, bool weno_w,
-                      const SweepK& K, const double (&eLb)[5], const double (&eRb)[5], double srcb,
-                      double (&stb)[5][ORD]) {
+// q0b = (d (eL, eR, src_avg) / d q0)^T (eLb, eRb, srcb): reverse of the
+// transform stage, recomputing its levels from the GLL values q0
+template <int NGLL, int DIR>
+AWFL_HD void cell_dt_adj(const double (&q0)[5][NGLL], const CellBg<NGLL>& bg, const SweepK& K,
+                         const double (&eLb)[5], const double (&eRb)[5], double srcb, double (&q0b)[5][NGLL]) {
   constexpr int NA = NGLL;
   constexpr int NM = 1 + DIR;
   Tape<NGLL, double> tp;
   double eL[5], eR[5], sa;
-  cell_fwd<ORD, NGLL, DIR>(st, bg, weno_s, weno_w, K, tp, eL, eR, sa);
+  cell_dt<NGLL, DIR>(q0, bg, K, tp, eL, eR, sa);
   const auto& q = tp.q;
   const auto& f = tp.f;
   const auto& g = tp.g;
@@ -661,11 +674,21 @@ AWFL_HD void cell_adj(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool w
     if (bg.wall_top) qb[idW][0][NGLL - 1] = 0.0;
     if (bg.wall_bot) qb[idW][0][0] = 0.0;
   }
+  sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { q0b[l][ii] = qb[l][0][ii]; }); });
+}
+
+// stb += (d (eL, eR, src_avg) / d st)^T (eLb, eRb, srcb) for one cell
+template <int ORD, int NGLL, int DIR>
+AWFL_HD void cell_adj(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s, bool weno_w,
+                      const SweepK& K, const double (&eLb)[5], const double (&eRb)[5], double srcb,
+                      double (&stb)[5][ORD]) {
+  double q0[5][NGLL], q0b[5][NGLL];
+  cell_recon<ORD, NGLL>(st, weno_s, weno_w, q0);
+  cell_dt_adj<NGLL, DIR>(q0, bg, K, eLb, eRb, srcb, q0b);
   sfor<5>([&](auto l) {
-    double gbv[NGLL], ub[ORD];
-    sfor<NGLL>([&](auto ii) { gbv[ii] = qb[l][0][ii]; });
+    double ub[ORD];
     sfor<ORD>([&](auto s) { ub[s] = 0.0; });
-    reconstruct_adj<ORD, NGLL>(st[l], (l == idR || l == idT) ? weno_s : weno_w, gbv, ub);
+    reconstruct_adj<ORD, NGLL>(st[l], (l == idR || l == idT) ? weno_s : weno_w, q0b[l], ub);
     sfor<ORD>([&](auto s) { stb[l][s] += ub[s]; });
   });
 }

@@ -20,7 +20,8 @@ def main():
     ap.add_argument("--ngll", type=int, default=3)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=512)
-    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "fp32", "adjoint"), default="fp64",
+                    help="fp64 sweeps, FP32 perturbation-form sweeps, or FP64 adjoint sweeps")
     a = ap.parse_args()
     import torch
     from awflcloud_amd import Dycore, bubble_spec
@@ -35,6 +36,11 @@ def main():
         for d in range(3):
             for _ in range(a.reps):
                 D.sweep(st, out, dt, d)
+    elif a.precision == "adjoint":
+        y = torch.randn_like(st)
+        for d in range(3):
+            for _ in range(a.reps):
+                D.sweep_adjoint(st, y, dt, d)
     else:
         s, c = D.to_f32(st)
         so, co = torch.empty_like(s), torch.empty_like(c)

@@ -32,6 +32,10 @@ BYTES_PER_CELL_SWEEP = 80.0  # read + write 5 doubles
 FLOPS_PER_CELL_SWEEP = 2412.0
 
 NX, NY, NZ = 256, 256, 128
+# HBM bytes (read + written) per launch of each FP64 sweep kernel at
+# 256x256x128, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes calibrated by a
+# copy kernel of known size (profiles/sweeps_fp64_256x256x128_hbm_traffic.md)
+TRAFFIC_BYTES = {0: 681.9e6, 1: 751.7e6, 2: 732.9e6}
 # FP32 perturbation-form comparison: 512^2 x 256 bubble + noise + band-limited
 # theta' field, 200 steps FP64 vs FP32 from the same initial state
 NX4, NY4, NZ4, STEPS4 = 512, 512, 256, 200
@@ -118,16 +122,18 @@ def fp32_leg(steps, warmup):
             "finite": bool(torch.isfinite(st32).all())}
 
 
-def weno9_leg(steps=50, warmup=2):
+def weno9_leg(steps=50, warmup=2, cfl=0.5):
     """WENO9/ADER5 FP64, 192x192x96, two opposite seeded bubbles: fused steps
-    through awfl_time_step, with the per-sweep mean kernel time."""
+    through awfl_time_step, with the per-sweep mean kernel time.  CFL 0.5:
+    the scheme with ngll = 5 is unstable at the driver's 0.8 (the oracle and
+    this path alike blow up within a few steps there)."""
     import torch
     from awflcloud_amd import Dycore
     from awflcloud_amd.workloads import two_bubbles_spec
     nx, ny, nz = 192, 192, 96
     D = Dycore(nx, ny, nz, bc=(0, 0, 1), ord=9, ngll=5)
     st = D.init_state(two_bubbles_spec(seed=3))
-    dt = D.compute_time_step(st, 0.8)
+    dt = D.compute_time_step(st, cfl)
     D.dim_switch = 1
     for _ in range(warmup):
         D.time_step(st, dt)
@@ -148,7 +154,7 @@ def weno9_leg(steps=50, warmup=2):
         torch.cuda.synchronize()
         kms["xyz"[d]] = e0.elapsed_time(e1) / 5
     return {"workload": "two opposite bubbles (+2 K r 1.5 km, -1.5 K r 2.5 km, seed 3) %dx%dx%d, WENO9/ADER5, "
-                        "FP64, %d steps" % (nx, ny, nz, steps),
+                        "FP64, %d steps, CFL %.1f" % (nx, ny, nz, steps, cfl),
             "cell_updates_per_s": nx * ny * nz * steps / wall, "ms_per_step": wall / steps * 1e3,
             "sweep_ms": kms, "finite": bool(torch.isfinite(st).all())}
 
@@ -280,7 +286,7 @@ def main():
         "config": {"workload": "thermal bubble 256x256x128, WENO5/ADER3, FP64, periodic x/y, wall z",
                    "nx": NX, "ny": NY, "nz": NZ, "ord": 5, "ngll": 3, "dt": dt, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES[dom],
                      "kernel": ["sweep_x", "sweep_y", "sweep_z"][dom],
                      "kernel_ms": {["x", "y", "z"][d]: kms[d] for d in kms}},
         "valu": {"achieved_tflops": flops, "peak_tflops": FP64_VALU_PEAK_TFS, "frac": flops / FP64_VALU_PEAK_TFS,

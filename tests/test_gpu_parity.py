@@ -238,13 +238,14 @@ def test_thermal_64x64x32_100_steps():
 
 def test_weno9_two_bubbles_steps():
     """WENO9/ADER5 on two opposite seeded bubbles (the high-order workload at
-    a size the oracle runs in seconds): 20 steps against the oracle."""
+    a size the oracle runs in seconds): 20 steps against the oracle at CFL
+    0.5 (at the driver's 0.8 this scheme is unstable, in the oracle too)."""
     _need_gpu()
     from awflcloud_amd.workloads import two_bubbles_spec
     D, O = _pair(48, 40, 32, ord=9, ngll=5, nthreads=8)
     st = D.init_state(two_bubbles_spec(seed=3))
     host = st.cpu().numpy().copy()
-    dt = D.compute_time_step(st, 0.8)
+    dt = D.compute_time_step(st, 0.5)
     O.set_dt_cache(dt)
     rbar, rtbar = _bg(O)
     O.dim_switch = 1

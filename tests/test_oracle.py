@@ -498,3 +498,26 @@ def test_noise_generator_is_uniform_and_seeded():
     assert abs(v.mean()) < 0.02 and abs(v.var() - 1 / 3) < 0.01
     w = np.array([pyoracle.uniform_pm1(12, i) for i in range(100)])
     assert not np.array_equal(v[:100], w)
+
+
+def test_weno9_ader5_stability_depends_on_cfl():
+    """A property of the reference scheme, recorded so that the WENO9/ADER5
+    workload is run where it is stable: with ngll = 5 the thermal bubble
+    blows up within a few steps at the driver's CFL 0.8 and stays bounded at
+    0.5 (48x40x32, 40 steps)."""
+    from helpers import bubble_dicts
+
+    def run(cfl):
+        O = pyoracle.Oracle(48, 40, 32, ord=9, ngll=5, nthreads=8)
+        st = O.init_state(pyoracle.make_spec(bubble_dicts()))
+        dt = O.compute_time_step(st, cfl)
+        O.dim_switch = 1
+        for s in range(40):
+            O.time_step(st, dt)
+            I = O.interior(st)
+            if not np.isfinite(I).all() or np.abs(I[3]).max() > 50:
+                return s + 1
+        return None
+
+    assert run(0.8) is not None
+    assert run(0.5) is None
