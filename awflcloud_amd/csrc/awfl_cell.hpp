@@ -167,8 +167,8 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   } else {
     // FP32 (and the tangent-linear scalar): products of the eps-sized TV
     // denominators would underflow, so the first normalisation takes one
-    // reciprocal per weight; the map's denominators d^2 + w (1 - 2d) are
-    // bounded below by min(d, 1-d)^2 and share one reciprocal
+    // reciprocal per weight; in FP32 the map's denominators d^2 + w (1 - 2d),
+    // bounded below by min(d, 1-d)^2, share one reciprocal
     sfor<hs + 2>([&](auto i) {
       constexpr T d = (T)W::idl[i];
       w[i] = d * rcp_t(tv[i] * tv[i] + (T)WENO_EPS);
@@ -177,25 +177,36 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
     sfor<hs + 2>([&](auto i) { sum += w[i]; });
     T rs = rcp_t(sum + (T)WENO_EPS);
     sfor<hs + 2>([&](auto i) { w[i] *= rs; });
-    T A[hs + 2], E[hs + 2], ex[hs + 2], P;
-    sfor<hs + 2>([&](auto i) {
-      constexpr T d = (T)W::idl[i];
-      constexpr T d2 = (T)(W::idl[i] + W::idl[i] * W::idl[i]);
-      constexpr T dd = (T)(W::idl[i] * W::idl[i]);
-      constexpr T c3 = (T)(3.0 * W::idl[i]);
-      constexpr T c12 = (T)(1.0 - 2.0 * W::idl[i]);
-      A[i] = w[i] * (d2 - c3 * w[i] + w[i] * w[i]);
-      E[i] = dd + w[i] * c12;
-      (void)d;
-    });
-    excl_products<hs + 2>(E, ex, P);
-    sum = 0;
-    sfor<hs + 2>([&](auto i) {
-      w[i] = A[i] * ex[i];
-      sum += w[i];
-    });
-    rs = rcp_t(sum + (T)WENO_EPS * P);
-    sfor<hs + 2>([&](auto i) { w[i] *= rs; });
+    if constexpr (std::is_same<T, float>::value) {
+      T A[hs + 2], E[hs + 2], ex[hs + 2], P;
+      sfor<hs + 2>([&](auto i) {
+        constexpr T d2 = (T)(W::idl[i] + W::idl[i] * W::idl[i]);
+        constexpr T dd = (T)(W::idl[i] * W::idl[i]);
+        constexpr T c3 = (T)(3.0 * W::idl[i]);
+        constexpr T c12 = (T)(1.0 - 2.0 * W::idl[i]);
+        A[i] = w[i] * (d2 - c3 * w[i] + w[i] * w[i]);
+        E[i] = dd + w[i] * c12;
+      });
+      excl_products<hs + 2>(E, ex, P);
+      sum = 0;
+      sfor<hs + 2>([&](auto i) {
+        w[i] = A[i] * ex[i];
+        sum += w[i];
+      });
+      rs = rcp_t(sum + (T)WENO_EPS * P);
+      sfor<hs + 2>([&](auto i) { w[i] *= rs; });
+    } else {
+      // the tangent-linear scalar: the map as written, the statements the
+      // hand-written reverse follows
+      sfor<hs + 2>([&](auto i) {
+        constexpr T d = (T)W::idl[i];
+        w[i] = w[i] * (d + d * d - (T)3 * d * w[i] + w[i] * w[i]) * rcp_t(d * d + w[i] * ((T)1 - (T)2 * d));
+      });
+      sum = 0;
+      sfor<hs + 2>([&](auto i) { sum += w[i]; });
+      rs = rcp_t(sum + (T)WENO_EPS);
+      sfor<hs + 2>([&](auto i) { w[i] *= rs; });
+    }
   }
   // weighted sum (:163-171); candidates have no coefficients above hs
   sfor<ORD>([&](auto ii) {
