@@ -50,13 +50,14 @@ __device__ __forceinline__ int64_t cidx(const Grid& G, const Pos& p) {
   return ((int64_t)p.k * G.ny + p.j) * G.nx + p.i;
 }
 __device__ __forceinline__ int64_t ncell(const Grid& G) { return (int64_t)G.nx * G.ny * G.nz; }
-__device__ __forceinline__ Pos pos_of(const Grid& G, int64_t t) {
-  Pos p;
-  p.i = (int)(t % G.nx);
-  int64_t r = t / G.nx;
-  p.j = (int)(r % G.ny);
-  p.k = (int)(r / G.ny);
-  return p;
+// kernels over cells launch a grid (ceil(nx/64), ny, nz) of 64-lane blocks:
+// the cell comes from the block and lane indices without any division
+__device__ __forceinline__ bool cell_of_launch(const Grid& G, Pos& p, int64_t& t) {
+  p.i = blockIdx.x * blockDim.x + threadIdx.x;
+  p.j = blockIdx.y;
+  p.k = blockIdx.z;
+  t = ((int64_t)p.k * G.ny + p.j) * G.nx + p.i;
+  return p.i < G.nx;
 }
 template <int DIR>
 __device__ __forceinline__ int along(const Pos& p) {
@@ -77,9 +78,10 @@ __device__ __forceinline__ int axis_len(const Grid& G) {
 // wall clamp); zn marks a clamped slot, whose normal momentum is zero
 __device__ __forceinline__ int map_axis(int a, int n, bool per, bool& zn) {
   zn = false;
-  if (per) {
-    int s = a % n;
-    return s < 0 ? s + n : s;
+  if (per) {                    // a is within a few cells of [0, n)
+    while (a < 0) a += n;
+    while (a >= n) a -= n;
+    return a;
   }
   if (a < 0) {
     zn = true;
@@ -100,6 +102,7 @@ __device__ __forceinline__ CellBg<NGLL> cell_bg(const Grid& G, int k) {
       bg.rG[ii] = G.hyDensGLL[k * NGLL + ii];
       bg.rtG[ii] = G.hyDensThetaGLL[k * NGLL + ii];
       bg.pG[ii] = G.hyPressureGLL[k * NGLL + ii];
+      bg.ptG[ii] = G.prG[k * NGLL + ii];
     });
     const bool per = G.bc[2] == AWFL_BC_PERIODIC;
     bg.wall_bot = !per && k == 0;
@@ -107,9 +110,27 @@ __device__ __forceinline__ CellBg<NGLL> cell_bg(const Grid& G, int k) {
   } else {
     bg.r = G.hyDensCells[G.hs + k];
     bg.rt = G.hyDensThetaCells[G.hs + k];
+    bg.pt = G.prT[k];
     bg.wall_bot = bg.wall_top = false;
   }
   return bg;
+}
+
+// power references of face f of the line through level k, as the forward
+// sweeps use them: x the level's (rho theta), y (rho rho*theta) and the
+// level's (rho theta), z the face's GLL value
+template <int DIR>
+__device__ __forceinline__ FaceRef face_ref(const Grid& G, int k, int f) {
+  FaceRef fr;
+  if constexpr (DIR == 0) {
+    fr.rt = fr.q5 = G.prT[k];
+  } else if constexpr (DIR == 1) {
+    fr.rt = G.prRT[k];
+    fr.q5 = G.prT[k];
+  } else {
+    fr.rt = fr.q5 = G.prF[f];
+  }
+  return fr;
 }
 
 // background pressure subtracted by the z flux at face f (S:1628-1629)
@@ -157,10 +178,10 @@ template <int ORD, int NGLL, int DIR, class T>
 __global__ __launch_bounds__(64) void k_edges(const double* __restrict__ st, const double* __restrict__ dst, Grid G,
                                               SweepK K, double* __restrict__ E, double* __restrict__ Ed,
                                               double* __restrict__ Q0) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  Pos p;
+  int64_t t;
+  if (!cell_of_launch(G, p, t)) return;
   const int64_t nc = ncell(G);
-  if (t >= nc) return;
-  const Pos p = pos_of(G, t);
   T w[5][ORD];
   if constexpr (std::is_same<T, double>::value) load_stencil<ORD, DIR>(st, G, p, w);
   else load_stencil<ORD, DIR>(st, dst, G, p, w);
@@ -216,10 +237,10 @@ template <int NGLL, int DIR>
 __global__ __launch_bounds__(64) void k_face_adj(const double* __restrict__ E, const double* __restrict__ adj_out,
                                                  Grid G, SweepK K, double* __restrict__ EB) {
   constexpr int NM = 1 + DIR;
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  Pos p;
+  int64_t t;
+  if (!cell_of_launch(G, p, t)) return;
   const int64_t nc = ncell(G);
-  if (t >= nc) return;
-  const Pos p = pos_of(G, t);
   const int n = axis_len<DIR>(G);
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   const int a = along<DIR>(p);
@@ -243,7 +264,7 @@ __global__ __launch_bounds__(64) void k_face_adj(const double* __restrict__ E, c
       if (has_l) v -= adj_out[sidx(G, l, pl)];
       Fb[l] = (G.sim2d && l == idV) ? 0.0 : c * v;
     });
-    adj::riemann_adj<DIR>(L, R, Fb, Lb, Rb);
+    adj::riemann_adj<DIR>(L, R, face_ref<DIR>(G, p.k, f), Fb, Lb, Rb);
     if (!per && f == 0) {
       sfor<5>([&](auto l) { EB[l * nc + cr] = Lb[l] + Rb[l]; });
     } else if (!per && f == n) {
@@ -266,10 +287,10 @@ template <int NGLL, int DIR>
 __global__ __launch_bounds__(64) void k_dt_adj(const double* __restrict__ Q0, const double* __restrict__ EB,
                                                const double* __restrict__ adj_out, Grid G, SweepK K,
                                                double* __restrict__ Q0B) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  Pos p;
+  int64_t t;
+  if (!cell_of_launch(G, p, t)) return;
   const int64_t nc = ncell(G);
-  if (t >= nc) return;
-  const Pos p = pos_of(G, t);
   double q0[5][NGLL], q0b[5][NGLL], eLb[5], eRb[5];
   sfor<5>([&](auto l) {
     sfor<NGLL>([&](auto ii) { q0[l][ii] = Q0[(l * NGLL + ii) * nc + t]; });
@@ -286,16 +307,19 @@ __global__ __launch_bounds__(64) void k_dt_adj(const double* __restrict__ Q0, co
 // WENO reverse, one thread per (variable, cell): GLL-value adjoints ->
 // stencil adjoints STB[l*ORD+s][cell]
 template <int ORD, int NGLL, int DIR>
-__global__ __launch_bounds__(256) void k_weno_adj(const double* __restrict__ st, const double* __restrict__ Q0B,
+__global__ __launch_bounds__(64) void k_weno_adj(const double* __restrict__ st, const double* __restrict__ Q0B,
                                                   Grid G, double* __restrict__ STB) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
+  // grid (ceil(nx/64), 5 ny, nz): the variable is blockIdx.y / ny
   const int64_t nc = ncell(G);
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= 5 * nc) return;
-  const int l = (int)(g / nc);
-  const int64_t t = g - (int64_t)l * nc;
-  const Pos p = pos_of(G, t);
+  const int l = blockIdx.y / G.ny;
+  Pos p;
+  p.i = blockIdx.x * blockDim.x + threadIdx.x;
+  p.j = blockIdx.y - l * G.ny;
+  p.k = blockIdx.z;
+  if (p.i >= G.nx) return;
+  const int64_t t = ((int64_t)p.k * G.ny + p.j) * G.nx + p.i;
   const int n = axis_len<DIR>(G);
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   const int a0 = along<DIR>(p);
@@ -311,20 +335,179 @@ __global__ __launch_bounds__(256) void k_weno_adj(const double* __restrict__ st,
   sfor<ORD>([&](auto s) { STB[(l * ORD + s) * nc + t] = ub[s]; });
 }
 
+// ---------------------------------------------------------------------------
+// x sweep adjoint fused into one kernel: one work-group per x-line with the
+// line, its adjoint and every intermediate in LDS (nothing per cell crosses
+// HBM).  LDS (doubles): state line [5][nx+2hs], adj_out line [5][nx], edge
+// states [10][nx], and a region [NB][nx] that holds the WENO values and the
+// edge-state adjoints of each cell and then, in place, its stencil adjoints.
+// ---------------------------------------------------------------------------
+template <int ORD, int NGLL>
+constexpr int adj_x_nb() {
+  return 5 * ORD > 5 * NGLL + 10 ? 5 * ORD : 5 * NGLL + 10;
+}
+
+template <int ORD, int NGLL>
+__global__ __launch_bounds__(256) void k_adj_x(const double* __restrict__ in, const double* __restrict__ adj_out,
+                                               double* __restrict__ adj_in, Grid G, SweepK K) {
+  constexpr int hs = (ORD - 1) / 2;
+  constexpr int Q0C = 0, EBC = 5 * NGLL;     // component offsets in the shared region
+  extern __shared__ double sm[];
+  const int nx = G.nx;
+  const int W = nx + 2 * hs;
+  const unsigned line = tile_index();
+  const int j = line % G.ny;
+  const int k = line / G.ny;
+  double* s_st = sm;
+  double* s_ao = s_st + 5 * W;
+  double* s_e = s_ao + 5 * nx;
+  double* s_b = s_e + 10 * nx;
+  const int64_t row = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs;
+  const bool per = G.bc[0] == AWFL_BC_PERIODIC;
+  // 1. stage the line (boundary rule at load) and its adjoint
+  for (int c = threadIdx.x; c < W; c += blockDim.x) {
+    bool zn;
+    const int src = map_axis(c - hs, nx, per, zn);
+    sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == idU) ? 0.0 : in[l * G.sl + row + src]; });
+  }
+  for (int c = threadIdx.x; c < nx; c += blockDim.x)
+    sfor<5>([&](auto l) { s_ao[l * nx + c] = adj_out[l * G.sl + row + c]; });
+  __syncthreads();
+  const CellBg<NGLL> bg = cell_bg<NGLL, 0>(G, k);
+  // 2. cell forward: WENO values and edge states
+  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
+    double w[5][ORD], q0[5][NGLL], eL[5], eR[5], sa;
+    sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { w[l][s] = s_st[l * W + c + s]; }); });
+    adj::cell_recon<ORD, NGLL>(w, G.weno_s, G.weno_w, q0);
+    sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { s_b[(Q0C + l * NGLL + ii) * nx + c] = q0[l][ii]; }); });
+    adj::Tape<NGLL, double> tp;
+    adj::cell_dt<NGLL, 0>(q0, bg, K, tp, eL, eR, sa);
+    sfor<5>([&](auto l) {
+      s_e[l * nx + c] = eL[l];
+      s_e[(5 + l) * nx + c] = eR[l];
+    });
+  }
+  __syncthreads();
+  // 3. faces: flux adjoint -> edge-state adjoints (each written by one face)
+  const double cdr = K.dt * K.rd;
+  const FaceRef fr = face_ref<0>(G, k, 0);
+  const int nfaces = per ? nx : nx + 1;
+  for (int f = threadIdx.x; f < nfaces; f += blockDim.x) {
+    const bool wl = !per && f == 0, wr = !per && f == nx;
+    const int a = wl ? 0 : (f == 0 ? nx - 1 : f - 1);      // cell on the left
+    const int b = wr ? nx - 1 : (f == nx ? 0 : f);         // cell on the right
+    double L[5], R[5], Fb[5], Lb[5], Rb[5];
+    sfor<5>([&](auto l) {
+      L[l] = wl ? s_e[l * nx + 0] : s_e[(5 + l) * nx + a];
+      R[l] = wr ? s_e[(5 + l) * nx + nx - 1] : s_e[l * nx + b];
+      double v = 0.0;
+      if (!wr) v += s_ao[l * nx + b];
+      if (!wl) v -= s_ao[l * nx + a];
+      Fb[l] = (G.sim2d && l == idV) ? 0.0 : cdr * v;
+      Lb[l] = Rb[l] = 0.0;
+    });
+    adj::riemann_adj<0>(L, R, fr, Fb, Lb, Rb);
+    if (wl) {
+      sfor<5>([&](auto l) { s_b[(EBC + l) * nx + 0] = Lb[l] + Rb[l]; });
+    } else if (wr) {
+      sfor<5>([&](auto l) { s_b[(EBC + 5 + l) * nx + nx - 1] = Lb[l] + Rb[l]; });
+    } else {
+      sfor<5>([&](auto l) {
+        s_b[(EBC + 5 + l) * nx + a] = Lb[l];
+        s_b[(EBC + l) * nx + b] = Rb[l];
+      });
+    }
+  }
+  __syncthreads();
+  // 4. cell reverse; the cell's stencil adjoints replace its own entries
+  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
+    double q0[5][NGLL], q0b[5][NGLL], eLb[5], eRb[5];
+    sfor<5>([&](auto l) {
+      sfor<NGLL>([&](auto ii) { q0[l][ii] = s_b[(Q0C + l * NGLL + ii) * nx + c]; });
+      eLb[l] = s_b[(EBC + l) * nx + c];
+      eRb[l] = s_b[(EBC + 5 + l) * nx + c];
+    });
+    adj::cell_dt_adj<NGLL, 0>(q0, bg, K, eLb, eRb, 0.0, q0b);
+    sfor<5>([&](auto l) {
+      double u[ORD], ub[ORD];
+      sfor<ORD>([&](auto s) {
+        u[s] = s_st[l * W + c + s];
+        ub[s] = 0.0;
+      });
+      adj::reconstruct_adj<ORD, NGLL>(u, (l == idR || l == idT) ? G.weno_s : G.weno_w, q0b[l], ub);
+      sfor<ORD>([&](auto s) { s_b[(l * ORD + s) * nx + c] = ub[s]; });
+    });
+  }
+  __syncthreads();
+  // 5. gather and write the line (its x halo cells are zero)
+  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
+    double acc[5];
+    sfor<5>([&](auto l) { acc[l] = s_ao[l * nx + c]; });
+    sfor<ORD>([&](auto s) {
+      int src = c + hs - s;            // the cell whose stencil slot s is c
+      bool ok = true;
+      if (per) {
+        while (src < 0) src += nx;
+        while (src >= nx) src -= nx;
+      } else {
+        ok = src >= 0 && src < nx;
+      }
+      if (ok) sfor<5>([&](auto l) { acc[l] += s_b[(l * ORD + s) * nx + src]; });
+    });
+    if (!per) {
+      if (c == 0)
+        sfor<hs>([&](auto cc) {
+          sfor<ORD>([&](auto s) {
+            if constexpr (cc - hs + s < 0) {
+              if (cc < nx) sfor<5>([&](auto l) {
+                if constexpr (l != idU) acc[l] += s_b[(l * ORD + s) * nx + cc];
+              });
+            }
+          });
+        });
+      if (c == nx - 1)
+        sfor<hs>([&](auto m) {
+          const int cc = nx - 1 - m;
+          sfor<ORD>([&](auto s) {
+            if constexpr (s >= hs + 1 + m) {
+              if (cc >= 0) sfor<5>([&](auto l) {
+                if constexpr (l != idU) acc[l] += s_b[(l * ORD + s) * nx + cc];
+              });
+            }
+          });
+        });
+    }
+    sfor<5>([&](auto l) { adj_in[l * G.sl + row + c] = acc[l]; });
+  }
+  for (int c = threadIdx.x; c < 2 * hs; c += blockDim.x) {
+    const int64_t o = c < hs ? row - hs + c : row + nx + (c - hs);
+    sfor<5>([&](auto l) { adj_in[l * G.sl + o] = 0.0; });
+  }
+}
+
+// zero the y and z halo rows of a padded array (the x halo cells of interior
+// rows are written by the line kernels)
+__global__ __launch_bounds__(64) void k_zero_halo_rows(Grid G, double* __restrict__ a) {
+  const int ih = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ih >= G.nx + 2 * G.hs) return;
+  const int j = (int)blockIdx.y - G.hs, k = (int)blockIdx.z - G.hs;
+  if (j >= 0 && j < G.ny && k >= 0 && k < G.nz) return;
+  const int64_t t = (int64_t)blockIdx.z * G.sz + (int64_t)blockIdx.y * G.sy + ih;
+  sfor<5>([&](auto l) { a[l * G.sl + t] = 0.0; });
+}
+
 // adj_in over the padded layout: zero in the halo, in the interior the
 // identity part plus every stencil adjoint that points at the cell
 template <int ORD, int DIR>
-__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ STB, const double* __restrict__ adj_out,
+__global__ __launch_bounds__(64) void k_gather(const double* __restrict__ STB, const double* __restrict__ adj_out,
                                                 Grid G, double* __restrict__ adj_in) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t nxh = G.nx + 2 * G.hs, nyh = G.ny + 2 * G.hs;
-  if (t >= G.sl) return;
-  const int ih = (int)(t % nxh);
-  const int jh = (int)((t / nxh) % nyh);
-  const int kh = (int)(t / (nxh * nyh));
-  const int i = ih - G.hs, j = jh - G.hs, k = kh - G.hs;
+  // grid (ceil((nx+2hs)/64), ny+2hs, nz+2hs) over the padded layout
+  const int ih = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ih >= G.nx + 2 * G.hs) return;
+  const int64_t t = (int64_t)blockIdx.z * G.sz + (int64_t)blockIdx.y * G.sy + ih;
+  const int i = ih - G.hs, j = (int)blockIdx.y - G.hs, k = (int)blockIdx.z - G.hs;
   if (i < 0 || i >= G.nx || j < 0 || j >= G.ny || k < 0 || k >= G.nz) {
     sfor<5>([&](auto l) { adj_in[l * G.sl + t] = 0.0; });
     return;
@@ -389,17 +572,15 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ STB, 
 
 // tangent-linear update over the padded layout (halo zero)
 template <int NGLL, int DIR>
-__global__ __launch_bounds__(256) void k_tlm_update(const double* __restrict__ in, const double* __restrict__ din,
+__global__ __launch_bounds__(64) void k_tlm_update(const double* __restrict__ in, const double* __restrict__ din,
                                                     const double* __restrict__ E, const double* __restrict__ Ed,
                                                     Grid G, SweepK K, double* __restrict__ out,
                                                     double* __restrict__ dout) {
   constexpr int NM = 1 + DIR;
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t nxh = G.nx + 2 * G.hs, nyh = G.ny + 2 * G.hs;
-  if (t >= G.sl) return;
-  const int i = (int)(t % nxh) - G.hs;
-  const int j = (int)((t / nxh) % nyh) - G.hs;
-  const int k = (int)(t / (nxh * nyh)) - G.hs;
+  const int ih = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ih >= G.nx + 2 * G.hs) return;
+  const int64_t t = (int64_t)blockIdx.z * G.sz + (int64_t)blockIdx.y * G.sy + ih;
+  const int i = ih - G.hs, j = (int)blockIdx.y - G.hs, k = (int)blockIdx.z - G.hs;
   if (i < 0 || i >= G.nx || j < 0 || j >= G.ny || k < 0 || k >= G.nz) {
     sfor<5>([&](auto l) { out[l * G.sl + t] = dout[l * G.sl + t] = 0.0; });
     return;
@@ -416,7 +597,7 @@ __global__ __launch_bounds__(256) void k_tlm_update(const double* __restrict__ i
       L[l] = Dual(E[(offl + l) * nc + cl], Ed[(offl + l) * nc + cl]);
       R[l] = Dual(E[(offr + l) * nc + cr], Ed[(offr + l) * nc + cr]);
     });
-    adj::riemann_fwd<DIR>(L, R, DIR == 2 ? face_pressure<NGLL>(G, f) : 0.0, F);
+    adj::riemann_fwd<DIR>(L, R, face_ref<DIR>(G, p.k, f), DIR == 2 ? face_pressure<NGLL>(G, f) : 0.0, F);
   };
   Dual F0[5], F1[5];
   flux(a, F0);
@@ -441,6 +622,8 @@ struct VariantA {
   void (*weno[3])(const double*, const double*, Grid, double*);
   void (*gather[3])(const double*, const double*, Grid, double*);
   void (*tlm[3])(const double*, const double*, const double*, const double*, Grid, SweepK, double*, double*);
+  void (*adj_x)(const double*, const double*, double*, Grid, SweepK);
+  int adj_x_nb;
 };
 
 #define AWFL_VARIANT_A(O, N)                                                                        \
@@ -451,7 +634,7 @@ struct VariantA {
         {k_dt_adj<N, 0>, k_dt_adj<N, 1>, k_dt_adj<N, 2>},                                           \
         {k_weno_adj<O, N, 0>, k_weno_adj<O, N, 1>, k_weno_adj<O, N, 2>},                            \
         {k_gather<O, 0>, k_gather<O, 1>, k_gather<O, 2>},                                           \
-        {k_tlm_update<N, 0>, k_tlm_update<N, 1>, k_tlm_update<N, 2>},                               \
+        {k_tlm_update<N, 0>, k_tlm_update<N, 1>, k_tlm_update<N, 2>}, k_adj_x<O, N>, adj_x_nb<O, N>(),  \
   }
 
 const VariantA kVariantsA[] = {AWFL_VARIANT_A(3, 2), AWFL_VARIANT_A(5, 3)};
@@ -472,7 +655,13 @@ int ensure_scratch(awfl_ctx* c) {
   return AWFL_OK;
 }
 
-unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+// launch shapes: cells (ceil(nx/64), ny, nz), (cell, variable) pairs
+// (ceil(nx/64), 5 ny, nz), padded layout (ceil((nx+2hs)/64), ny+2hs, nz+2hs)
+dim3 cells_grid(const awfl_ctx* c) { return dim3((c->cfg.nx + 63) / 64, c->cfg.ny, c->cfg.nz); }
+dim3 cellvar_grid(const awfl_ctx* c) { return dim3((c->cfg.nx + 63) / 64, 5 * c->cfg.ny, c->cfg.nz); }
+dim3 padded_grid(const awfl_ctx* c) {
+  return dim3((c->cfg.nx + 2 * c->hs + 63) / 64, c->cfg.ny + 2 * c->hs, c->cfg.nz + 2 * c->hs);
+}
 
 // adj_in = adj_out + J(in)^T (dt adj_out) for one sweep
 int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* adj_in, double dt, int dir) {
@@ -488,20 +677,34 @@ int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* 
   const SweepK K = sweep_constants(c, dt, dir);
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   const int ng = c->cfg.ngll;
+  if (dir == 0) {
+    const int nx = c->cfg.nx;
+    const size_t lds = sizeof(double) * (size_t)(5 * (nx + 2 * c->hs) + (15 + v->adj_x_nb) * nx);
+    if (lds <= 160 * 1024) {
+      if (lds > 64 * 1024)
+        HIP_TRY(c, hipFuncSetAttribute((const void*)v->adj_x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      const int threads = std::min(256, (nx + 63) / 64 * 64);
+      v->adj_x<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, adj_out, adj_in, G, K);
+      LAUNCH_CHECK(c);
+      k_zero_halo_rows<<<padded_grid(c), 64, 0, c->stream>>>(G, adj_in);
+      LAUNCH_CHECK(c);
+      return AWFL_OK;
+    }
+  }
   double* E = c->a_scr;
   double* EB = E + NE * nc;
   double* STB = EB + NEB * nc;
   double* Q0 = STB + (int64_t)5 * c->cfg.ord * nc;
   double* Q0B = Q0 + (int64_t)5 * ng * nc;
-  v->edges[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(in, nullptr, G, K, E, nullptr, Q0);
+  v->edges[dir]<<<cells_grid(c), 64, 0, c->stream>>>(in, nullptr, G, K, E, nullptr, Q0);
   LAUNCH_CHECK(c);
-  v->face[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(E, adj_out, G, K, EB);
+  v->face[dir]<<<cells_grid(c), 64, 0, c->stream>>>(E, adj_out, G, K, EB);
   LAUNCH_CHECK(c);
-  v->dt[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(Q0, EB, adj_out, G, K, Q0B);
+  v->dt[dir]<<<cells_grid(c), 64, 0, c->stream>>>(Q0, EB, adj_out, G, K, Q0B);
   LAUNCH_CHECK(c);
-  v->weno[dir]<<<blocks_for(5 * nc, 256), 256, 0, c->stream>>>(in, Q0B, G, STB);
+  v->weno[dir]<<<cellvar_grid(c), 64, 0, c->stream>>>(in, Q0B, G, STB);
   LAUNCH_CHECK(c);
-  v->gather[dir]<<<blocks_for(G.sl, 256), 256, 0, c->stream>>>(STB, adj_out, G, adj_in);
+  v->gather[dir]<<<padded_grid(c), 64, 0, c->stream>>>(STB, adj_out, G, adj_in);
   LAUNCH_CHECK(c);
   return AWFL_OK;
 }
@@ -593,9 +796,9 @@ int awfl_sweep_tangent(awfl_ctx* c, const double* d_in, const double* d_din, dou
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   double* E = c->a_scr;
   double* Ed = E + NE * nc;            // fits: NE <= NEB + 5*ord
-  v->edges_d[dir]<<<blocks_for(nc, 64), 64, 0, c->stream>>>(d_in, d_din, G, K, E, Ed, nullptr);
+  v->edges_d[dir]<<<cells_grid(c), 64, 0, c->stream>>>(d_in, d_din, G, K, E, Ed, nullptr);
   LAUNCH_CHECK(c);
-  v->tlm[dir]<<<blocks_for(G.sl, 256), 256, 0, c->stream>>>(d_in, d_din, E, Ed, G, K, d_out, d_dout);
+  v->tlm[dir]<<<padded_grid(c), 64, 0, c->stream>>>(d_in, d_din, E, Ed, G, K, d_out, d_dout);
   LAUNCH_CHECK(c);
   return AWFL_OK;
 }

@@ -55,6 +55,13 @@ AWFL_HD Dual pow_g(const Dual& x) {
   double p = pow_gamma(x.v);
   return Dual(p, GAMMA * p * rcp(x.v) * x.d);
 }
+// x^GAMMA near a background value B with known power (PowRef, as the
+// production kernels): B^GAMMA (1 + (x-B)/B)^GAMMA
+AWFL_HD double pow_at(double x, const PowRef& pr) { return pow_gamma_near(x, (x - pr.b) * pr.rb, pr.h, pr.l); }
+AWFL_HD Dual pow_at(const Dual& x, const PowRef& pr) {
+  double p = pow_gamma_near(x.v, (x.v - pr.b) * pr.rb, pr.h, pr.l);
+  return Dual(p, GAMMA * p * rcp(x.v) * x.d);
+}
 AWFL_HD double sqrt_g(double x) { return sqrt(x); }
 AWFL_HD Dual sqrt_g(const Dual& x) {
   double s = sqrt(x.v);
@@ -113,7 +120,7 @@ AWFL_HD void cell_dt(const T (&q0)[5][NGLL], const CellBg<NGLL>& bg, const Sweep
   sfor<NGLL>([&](auto ii) {
     tp.rr[ii] = rcp_g(q[idR][0][ii]);
     sfor<4>([&](auto m) { f[m][0][ii] = q[NM][0][ii] * q[m + 1][0][ii] * tp.rr[ii]; });
-    g[0][ii] = pow_g(q[idT][0][ii]);
+    g[0][ii] = pow_at(q[idT][0][ii], DIR == 2 ? bg.ptG[ii] : bg.pt);
     tp.rrt[ii] = rcp_g(q[idT][0][ii]);
     if constexpr (DIR == 2) sv = -(q[idR][0][ii] - T(bg.rG[ii])) * T(GRAV);
   });
@@ -215,7 +222,7 @@ AWFL_HD void cell_fwd(const T (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s
 // Upwind characteristic flux, generic scalar (statements of riemann in
 // awfl_cell.hpp); p_face is the background pressure the z flux subtracts.
 template <int DIR, class T>
-AWFL_HD void riemann_fwd(const T (&L)[5], const T (&R)[5], double p_face, T (&F)[5]) {
+AWFL_HD void riemann_fwd(const T (&L)[5], const T (&R)[5], const FaceRef& fr, double p_face, T (&F)[5]) {
   constexpr int NM = 1 + DIR;
   constexpr int T1 = DIR == 0 ? 2 : 1;
   constexpr int T2 = DIR == 2 ? 2 : 3;
@@ -233,7 +240,7 @@ AWFL_HD void riemann_fwd(const T (&L)[5], const T (&R)[5], double p_face, T (&F)
     ut2 = T(0.5) * (L[T2] * iL + R[T2] * iR);
     t = T(0.5) * (L[4] * iL + R[4] * iR);
   }
-  T p = T(C0) * pow_g(r * t);
+  T p = T(C0) * pow_at(r * t, fr.rt);
   T cs = sqrt_g(T(GAMMA) * p * rcp_g(r));
   T it = rcp_g(t);
   T i2c = T(0.5) * rcp_g(cs);
@@ -258,7 +265,7 @@ AWFL_HD void riemann_fwd(const T (&L)[5], const T (&R)[5], double p_face, T (&F)
   T q5 = t * w5 + t * w6;
   T iq1 = rcp_g(q1);
   F[0] = qn;
-  F[NM] = qn * qn * iq1 + T(C0) * pow_g(q5);
+  F[NM] = qn * qn * iq1 + T(C0) * pow_at(q5, fr.q5);
   if constexpr (DIR == 2) F[NM] -= T(p_face);
   F[T1] = qn * qt1 * iq1;
   F[T2] = qn * qt2 * iq1;
@@ -271,8 +278,8 @@ AWFL_HD void riemann_fwd(const T (&L)[5], const T (&R)[5], double p_face, T (&F)
 
 // Lb += (dF/dL)^T Fb, Rb += (dF/dR)^T Fb at the state (L, R)
 template <int DIR>
-AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const double (&Fb)[5], double (&Lb)[5],
-                         double (&Rb)[5]) {
+AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const FaceRef& fr, const double (&Fb)[5],
+                         double (&Lb)[5], double (&Rb)[5]) {
   constexpr int NM = 1 + DIR;
   constexpr int T1 = DIR == 0 ? 2 : 1;
   constexpr int T2 = DIR == 2 ? 2 : 3;
@@ -293,7 +300,7 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const doubl
     t = 0.5 * (L[4] * iL + R[4] * iR);
   }
   double rt = r * t;
-  double p = C0 * pow_gamma(rt);
+  double p = C0 * pow_at(rt, fr.rt);
   double ir = rcp(r);
   double cs2 = GAMMA * p * ir;
   double cs = sqrt(cs2);
@@ -315,7 +322,7 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const doubl
   double qt2 = w3 + ut2 * s56;
   double q5 = t * s56;
   double iq1 = rcp(q1);
-  double p5 = C0 * pow_gamma(q5);
+  double p5 = C0 * pow_at(q5, fr.q5);
   // reverse
   double qnb = Fb[0] + Fb[NM] * 2.0 * qn * iq1 + Fb[T1] * qt1 * iq1 + Fb[T2] * qt2 * iq1 + Fb[4] * q5 * iq1;
   double iq1b = Fb[NM] * qn * qn + Fb[T1] * qn * qt1 + Fb[T2] * qn * qt2 + Fb[4] * qn * q5;
@@ -451,11 +458,12 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   sfor<hs + 1>([&](auto i) { lo += tv[i]; });
   lo *= 1.0 / (hs + 1);
   tv[hs + 1] = lo + (tvhi - lo) * W::sigma;
-  double Dn[NP], b[NP], w1[NP], m[NP], w[NP];
+  double rDn[NP], b[NP], w1[NP], m[NP], w[NP], rden[NP];
   double S1 = WENO_EPS;
   sfor<NP>([&](auto i) {
-    Dn[i] = tv[i] * tv[i] + WENO_EPS;
-    b[i] = W::idl[i] * rcp(Dn[i]);
+    constexpr double d = W::idl[i];
+    rDn[i] = rcp(tv[i] * tv[i] + WENO_EPS);
+    b[i] = d * rDn[i];
     S1 += b[i];
   });
   const double R1 = rcp(S1);
@@ -463,7 +471,8 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   sfor<NP>([&](auto i) {
     constexpr double d = W::idl[i];
     w1[i] = b[i] * R1;
-    m[i] = w1[i] * (d + d * d - 3.0 * d * w1[i] + w1[i] * w1[i]) * rcp(d * d + w1[i] * (1.0 - 2.0 * d));
+    rden[i] = rcp(d * d + w1[i] * (1.0 - 2.0 * d));
+    m[i] = w1[i] * (d + d * d - 3.0 * d * w1[i] + w1[i] * w1[i]) * rden[i];
     S2 += m[i];
   });
   const double R2 = rcp(S2);
@@ -499,11 +508,9 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
     constexpr double d = W::idl[i];
     double mb = R2 * (wb[i] - dot);
     double x = w1[i];
-    double num = x * (d + d * d - 3.0 * d * x + x * x);
-    double den = d * d + x * (1.0 - 2.0 * d);
+    // m = num / den: m' = (num' - m (1 - 2d)) / den
     double dnum = d + d * d - 6.0 * d * x + 3.0 * x * x;
-    double iden = rcp(den);
-    w1b[i] = mb * (dnum * den - num * (1.0 - 2.0 * d)) * iden * iden;
+    w1b[i] = mb * (dnum - m[i] * (1.0 - 2.0 * d)) * rden[i];
   });
   // first normalisation w1 = b / (sum b + eps), b = idl / (tv^2 + eps)
   double dot1 = 0.0;
@@ -511,7 +518,7 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   double tvb[NP];
   sfor<NP>([&](auto i) {
     double bb = R1 * (w1b[i] - dot1);
-    tvb[i] = -2.0 * bb * b[i] * tv[i] * rcp(Dn[i]);
+    tvb[i] = -2.0 * bb * b[i] * tv[i] * rDn[i];
   });
   // sigma blend of the high-order indicator
   double tvhib = tvb[hs + 1] * W::sigma;

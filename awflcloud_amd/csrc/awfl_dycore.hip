@@ -240,10 +240,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     double eL[5], sa_;
     cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
   }
+  // software pipeline: the stencil cell of the next iteration is loaded one
+  // iteration ahead
+  double nxt[5];
+  sfor<5>([&](auto l) { nxt[l] = ld(l, s0 + hs); });
   for (int v = s0; v < s1; v++) {
     sfor<5>([&](auto l) {
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
-      w[l][ORD - 1] = ld(l, v + hs);
+      w[l][ORD - 1] = nxt[l];
+      nxt[l] = ld(l, v + 1 + hs);
     });
     CellBg<NGLL> bg;
     bg_of(v, bg);
@@ -271,7 +276,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
   } else {
     sfor<5>([&](auto l) {
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
-      w[l][ORD - 1] = ld(l, s1 + hs);
+      w[l][ORD - 1] = nxt[l];          // cell s1 + hs, loaded by the last iteration
     });
     CellBg<NGLL> bg;
     bg_of(s1 % n, bg);

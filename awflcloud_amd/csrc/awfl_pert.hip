@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_f32(const float* __restrict__ i
   float* s_st = smf;
   float* s_L = smf + 5 * W;
   float* s_R = s_L + 5 * nx;
-  float* s_F = s_R + 5 * nx;           // own region: the staged s is still read at the update
+  float* s_F = smf;                    // reuses the staging region
   const int64_t row = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs;
   const bool per = G.bc[0] == AWFL_BC_PERIODIC;
   for (int c = threadIdx.x; c < W; c += blockDim.x) {
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_f32(const float* __restrict__ i
       float td = -(s_F[l * (nx + 1) + cp] - s_F[l * (nx + 1) + c]) * K.rd;
       if (G.sim2d && l == idV) td = 0.0f;
       const int64_t o = l * G.sl + row + c;
-      kahan(s_st[l * W + c + hs], cin[o], K.dt * td, out[o], cout[o]);
+      kahan(in[o], cin[o], K.dt * td, out[o], cout[o]);   // s again: an L2 hit, and LDS stays small
     }
   }
 }
@@ -181,14 +181,15 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     }
   };
   auto face_bg = [&](int f) -> FaceBgP { return DIR == 1 ? G.fbL[o] : G.fbF[f]; };
-  auto emit = [&](int c, const float (&F0)[5], const float (&F1)[5], float src, const float (&sc)[5]) {
+  auto emit = [&](int c, const float (&F0)[5], const float (&F1)[5], float src, const float (&sc)[5],
+                  const float (&cc)[5]) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       float td = -(F1[l] - F0[l]) * K.rd;
       if (DIR == 2 && l == idW) td = src + td;
       if (DIR == 2 && G.sim2d && l == idV) td = 0.0f;
       const int64_t idx = l * G.sl + base + (int64_t)c * sa;
-      kahan(sc[l], cin[idx], K.dt * td, out[idx], cout[idx]);
+      kahan(sc[l], cc[l], K.dt * td, out[idx], cout[idx]);
     }
   };
 
@@ -201,13 +202,23 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     float eL[5], sa_;
     cell_edges_pert<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
   }
+  // software pipeline: the stencil cell of the next iteration and the
+  // compensation of the next update are loaded one iteration ahead
+  float nxt[5], cnx[5], cprev[5];
+  sfor<5>([&](auto l) {
+    nxt[l] = ld(l, s0 + hs);
+    cnx[l] = cin[l * G.sl + base + (int64_t)(((s0 - 1) % n + n) % n) * sa];
+  });
   for (int v = s0; v < s1; v++) {
     // the window is centred on cell v-1, the cell updated in this iteration
     float prevS[5];
     sfor<5>([&](auto l) {
       prevS[l] = w[l][hs];
+      cprev[l] = cnx[l];
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
-      w[l][ORD - 1] = ld(l, v + hs);
+      w[l][ORD - 1] = nxt[l];
+      nxt[l] = ld(l, v + 1 + hs);
+      cnx[l] = cin[l * G.sl + base + (int64_t)v * sa];
     });
     CellBgP<NGLL> bg;
     bg_of(v, bg);
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     const FaceBgP fb = face_bg(v);
     if (!per && v == 0) riemann_pert<DIR>(eL, eL, fb, F);
     else riemann_pert<DIR>(prevR, eL, fb, F);
-    if (v > s0) emit(v - 1, prevF, F, prevSrc, prevS);
+    if (v > s0) emit(v - 1, prevF, F, prevSrc, prevS, cprev);
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       prevF[l] = F[l];
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
   } else {
     sfor<5>([&](auto l) {
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
-      w[l][ORD - 1] = ld(l, s1 + hs);
+      w[l][ORD - 1] = nxt[l];          // cell s1 + hs, loaded by the last iteration
     });
     CellBgP<NGLL> bg;
     bg_of(s1 % n, bg);
@@ -241,7 +252,7 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     cell_edges_pert<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
     riemann_pert<DIR>(prevR, eL, fb, F);
   }
-  emit(s1 - 1, prevF, F, prevSrc, lastS);
+  emit(s1 - 1, prevF, F, prevSrc, lastS, cnx);
 }
 
 __global__ void k_to_f32(const double* __restrict__ d, float* __restrict__ s, float* __restrict__ c, int64_t n) {
@@ -377,7 +388,7 @@ int launch_sweep_f32(awfl_ctx* c, const float* s, const float* cs, float* so, fl
   if (dir == 0) {
     int nx = c->cfg.nx;
     int threads = std::min(256, (nx + 63) / 64 * 64);
-    size_t lds = sizeof(float) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx + 5 * (nx + 1));
+    size_t lds = sizeof(float) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx);
     if (lds > 64 * 1024)
       HIP_TRY(c, hipFuncSetAttribute((const void*)v->sweep_x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     v->sweep_x<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(s, cs, so, co, F, K);

@@ -173,15 +173,18 @@ static void test_riemann(Rng& R) {
       Rd[l] = R.uni(-1, 1) * (l == 0 ? 1e-3 : 1.0);
     }
     double pface = C0 * std::pow(rt0, GAMMA);
+    FaceRef fr;
+    fr.rt = power_ref(DIR == 1 ? r0 * rt0 : rt0);
+    fr.q5 = power_ref(rt0);
     Dual LD[5], RD[5], FD[5];
     for (int l = 0; l < 5; l++) {
       LD[l] = Dual(L[l], Ld[l]);
       RD[l] = Dual(Rs[l], Rd[l]);
     }
-    adj::riemann_fwd<DIR>(LD, RD, pface, FD);
+    adj::riemann_fwd<DIR>(LD, RD, fr, pface, FD);
     double Fb[5], Lb[5] = {}, Rb[5] = {};
     for (int l = 0; l < 5; l++) Fb[l] = R.uni(-1, 1);
-    adj::riemann_adj<DIR>(L, Rs, Fb, Lb, Rb);
+    adj::riemann_adj<DIR>(L, Rs, fr, Fb, Lb, Rb);
     double lhs = 0, lscale = 0, rhs = 0;
     for (int l = 0; l < 5; l++) {
       lhs += Fb[l] * FD[l].d;
@@ -189,11 +192,8 @@ static void test_riemann(Rng& R) {
       rhs += Lb[l] * Ld[l] + Rb[l] * Rd[l];
     }
     check("dot", 0, 0, DIR, "riemann", std::fabs(lhs - rhs) / lscale, 1e-12);
-    // production flux (with its power references) vs the generic primal and
-    // central differences
-    FaceRef fr;
-    fr.rt = power_ref(DIR == 1 ? r0 * rt0 : rt0);
-    fr.q5 = power_ref(rt0);
+    // production flux (with the same power references) vs the generic primal
+    // and central differences
     double F[5];
     riemann<DIR>(L, Rs, fr, F);
     if (DIR == 2) F[3] -= pface;
