@@ -14,8 +14,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 arithmetic stays off in every HIP build of this package.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-# translation units of libawfl_dycore.so: FP64 sweeps + ABI, FP32 perturbation form, adjoint
-HIP_UNITS = ["awfl_dycore.hip", "awfl_pert.hip", "awfl_adjoint.hip"]
+# translation units of libawfl_dycore.so: FP64 sweeps + ABI, FP32 perturbation form, adjoint,
+# alternative time integrators
+HIP_UNITS = ["awfl_dycore.hip", "awfl_pert.hip", "awfl_adjoint.hip", "awfl_splitting.hip"]
 
 
 def run(cmd, **kw):

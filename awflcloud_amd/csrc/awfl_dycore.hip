@@ -715,7 +715,7 @@ void awfl_destroy(awfl_ctx* c) {
   if (c->d_bgf) (void)hipFree(c->d_bgf);
   for (float* b : c->f_buf)
     if (b) (void)hipFree(b);
-  for (double* b : {c->a_scr, c->a_tmp, c->a_states})
+  for (double* b : {c->a_scr, c->a_tmp, c->a_states, c->r_tend[0], c->r_tend[1]})
     if (b) (void)hipFree(b);
   delete c;
 }
@@ -892,6 +892,10 @@ static int fill(awfl_ctx* c, double* p, int64_t n, double v) {
   k_fill<<<blocks, 256, 0, c->stream>>>(p, n, v);
   LAUNCH_CHECK(c);
   return AWFL_OK;
+}
+
+int launch_sweep_fp64(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {
+  return launch_sweep(c, in, out, dt, dir, tend);
 }
 
 extern "C" {

@@ -101,6 +101,8 @@ struct awfl_ctx {
   double* a_tmp = nullptr;
   double* a_states = nullptr;
   int64_t a_states_n = 0;
+  // alternative integrators (awfl_splitting.hip): two tendency-sized arrays
+  double* r_tend[2] = {nullptr, nullptr};
   int n_red = 0;
   std::string err;
   Grid grid() const;
@@ -148,6 +150,10 @@ inline awfl::SweepK sweep_constants(const awfl_ctx* c, double dt, int dir) {
   }
   return K;
 }
+
+// one FP64 sweep along dir (awfl_dycore.hip): out = in + dt * tend, or the
+// tendency itself (tend = true, out is tendency-sized)
+int launch_sweep_fp64(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend);
 
 inline int fail(awfl_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;

@@ -119,6 +119,8 @@ def lib():
         L.awfl_state_from_f32.argtypes = [P, P, P, P]
         L.awfl_sweep_f32.argtypes = [P, P, P, P, P, D, C.c_int]
         L.awfl_time_step_f32.argtypes = [P, P, P, D]
+        L.awfl_time_step_strang.argtypes = [P, P, D]
+        L.awfl_time_step_ssprk3.argtypes = [P, P, D]
         L.awfl_sweep_tangent.argtypes = [P, P, P, P, P, D, C.c_int]
         L.awfl_sweep_adjoint.argtypes = [P, P, P, P, D, C.c_int]
         L.awfl_time_step_adjoint.argtypes = [P, P, P, D, C.c_int]
@@ -246,6 +248,16 @@ class Dycore:
         for spl in range(self.num_split()):
             self.compute_tendencies(state, tend, dt, spl)
             self.apply_tendencies(state, tend, dt, spl)
+
+    def time_step_strang(self, state, dt):
+        """x, y with dt and z as two dt/2 sweeps (qjrmsCode/TimeIntegrator.cpp:32-65)"""
+        self._chk_state(state)
+        self._check(self.L.awfl_time_step_strang(self.h, self._ptr(state), dt))
+
+    def time_step_ssprk3(self, state, dt):
+        """SSPRK3 on the semi-discrete operator (qjrmsCode/TimeIntegrator.cpp:68-110)"""
+        self._chk_state(state)
+        self._check(self.L.awfl_time_step_ssprk3(self.h, self._ptr(state), dt))
 
     def sweep(self, src, dst, dt, direction):
         self._chk_state(src)

@@ -395,4 +395,37 @@ class Temporal_ader {
   const char* getTemporalName() { return "ADER-DT"; }
 };
 
+// The alternative integrators of qjrmsCode/TimeIntegrator.cpp over the same
+// Spatial operator, with the Temporal interface of Temporal_ader:
+//   Temporal_strang: x, y sweeps with dt and z as two dt/2 sweeps (:32-65)
+//   Temporal_ssprk3: SSPRK3 on the semi-discrete operator (:68-110)
+template <class Spatial, int (*STEP)(awfl_ctx*, double*, double)>
+class Temporal_alt {
+ public:
+  typedef typename Spatial::StateArr StateArr;
+  typedef typename Spatial::TracerArr TracerArr;
+  typedef typename Spatial::Location Location;
+
+  Spatial spaceOp;
+
+  void init(std::string inFile) { spaceOp.init(inFile); }
+  void timeStep(StateArr& stateArr, TracerArr&, real dt) {
+    if (STEP(spaceOp.context(), stateArr.data(), dt) < 0)
+      endrun(std::string("time step: ") + awfl_last_error(spaceOp.context()));
+  }
+  void finalize(StateArr& state, TracerArr& tracers) { spaceOp.finalize(state, tracers); }
+};
+
+template <class Spatial>
+class Temporal_strang : public Temporal_alt<Spatial, awfl_time_step_strang> {
+ public:
+  const char* getTemporalName() { return "ADER-DT, z as two half steps"; }
+};
+
+template <class Spatial>
+class Temporal_ssprk3 : public Temporal_alt<Spatial, awfl_time_step_ssprk3> {
+ public:
+  const char* getTemporalName() { return "SSPRK3, semi-discrete"; }
+};
+
 }  // namespace awfl

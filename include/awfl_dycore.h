@@ -115,6 +115,16 @@ int awfl_time_step(awfl_ctx* ctx, double* d_state, double dt);
  * d_in and d_out must not alias. */
 int awfl_sweep(awfl_ctx* ctx, const double* d_in, double* d_out, double dt, int dir);
 
+/* Alternative time integrators over the same spatial operator, behind the
+ * Temporal interface (qjrmsCode/TimeIntegrator.cpp; not used by the Agrid
+ * driver).  awfl_time_step_strang: x and y sweeps with dt, then z as two
+ * sweeps with dt/2 (:32-65), the reverse order on alternate steps (dimSwitch).
+ * awfl_time_step_ssprk3: three-stage SSP Runge-Kutta (:68-110) on the
+ * semi-discrete operator, the sum of the directional tendencies without the
+ * ADER time average.  Both in place on d_state; context-owned buffers. */
+int awfl_time_step_strang(awfl_ctx* ctx, double* d_state, double dt);
+int awfl_time_step_ssprk3(awfl_ctx* ctx, double* d_state, double dt);
+
 /* dimSwitch of S:93 (checkpoint / restart). */
 int awfl_get_dim_switch(const awfl_ctx* ctx);
 int awfl_set_dim_switch(awfl_ctx* ctx, int v);

@@ -170,6 +170,47 @@ class Oracle:
     def time_step(self, state, dt):
         self.L.oracle_time_step(self.h, _dp(state), dt)
 
+    # --- alternative time integrators over the same spatial operator ---
+    def sweep(self, state, dt, direction):
+        """One split sweep in place: interior += dt * tendency of `direction`
+        (the update of Temporal_ader.h:63-65 for one split)."""
+        tend = self.tendencies_dir(state, dt, direction)
+        self.interior(state)[...] += dt * tend
+
+    def strang_step(self, state, dt, ds):
+        """qjrmsCode/TimeIntegrator.cpp:32-65: x, y and then z as two half
+        steps when ds is set, the reverse order otherwise; y is skipped when
+        ny == 1 (run2d)."""
+        seq = [(0, dt), (1, dt), (2, dt / 2), (2, dt / 2)]
+        if not ds:
+            seq = seq[::-1]
+        for d, h in seq:
+            if d == 1 and self.ny == 1:
+                continue
+            self.sweep(state, h, d)
+
+    def semi_discrete_tendency(self, state):
+        """Sum of the three directional tendencies without the ADER time
+        average (dt = 0 in the transform: only the reconstructed values reach
+        the faces), as the semi-discrete operator of an explicit RK method."""
+        tend = self.tendencies_dir(state, 0.0, 0)
+        if self.ny > 1:
+            tend += self.tendencies_dir(state, 0.0, 1)
+        tend += self.tendencies_dir(state, 0.0, 2)
+        return tend
+
+    def ssprk3_step(self, state, dt):
+        """qjrmsCode/TimeIntegrator.cpp:68-96 with applyTendencies of :99-110
+        (state2 = c0 state0 + c1 state1 + ct dt tend, in that order)."""
+        I = self.interior
+        tmp = state.copy()
+        t = self.semi_discrete_tendency(state)
+        I(tmp)[...] = 1.0 * I(state) + 0.0 * I(tmp) + 1.0 * dt * t
+        t = self.semi_discrete_tendency(tmp)
+        I(tmp)[...] = 0.75 * I(state) + 0.25 * I(tmp) + 0.25 * dt * t
+        t = self.semi_discrete_tendency(tmp)
+        I(state)[...] = (1.0 / 3.0) * I(state) + (2.0 / 3.0) * I(tmp) + (2.0 / 3.0) * dt * t
+
     def run_steps(self, state, dt, nsteps):
         return self.L.oracle_run_steps(self.h, _dp(state), dt, nsteps)
 
