@@ -4,7 +4,10 @@
 // awfl_forward_ckpt / awfl_adjoint_ckpt in include/awfl_dycore.h).
 //
 // One split sweep is out = in + dt * T(in) (awfl_sweep).  Its adjoint
-// adj_in = adj_out + J^T (dt adj_out) runs as five kernels over per-cell
+// adj_in = adj_out + J^T (dt adj_out) runs as one fused kernel per line
+// along the sweep direction (k_adj_line: forward, face adjoints, transform
+// and WENO reverses and the gather, all in LDS).  Lines too long for LDS
+// (more than about 680 cells at WENO5) run as five kernels over per-cell
 // scratch in HBM:
 //   k_edges      cell forward, one thread per cell: WENO values at the GLL
 //                points [5 ngll/cell], then time-averaged edge states and
@@ -336,162 +339,203 @@ __global__ __launch_bounds__(64) void k_weno_adj(const double* __restrict__ st, 
 }
 
 // ---------------------------------------------------------------------------
-// x sweep adjoint fused into one kernel: one work-group per x-line with the
-// line, its adjoint and every intermediate in LDS (nothing per cell crosses
-// HBM).  LDS (doubles): state line [5][nx+2hs], adj_out line [5][nx], edge
-// states [10][nx], and a region [NB][nx] that holds the WENO values and the
-// edge-state adjoints of each cell and then, in place, its stencil adjoints.
+// Sweep adjoint fused into one kernel: one work-group per line along DIR
+// with the line and every intermediate in LDS (nothing per cell crosses
+// HBM).  x lines are contiguous; y and z lines are strided, and the
+// XCD-aware line order keeps neighbouring lines (which share every cache
+// line they touch) on one XCD at about the same time.  LDS (doubles): the
+// state line [5][n+2hs], a region [5 NGLL][n] for the WENO values of each
+// cell and then, in place, their adjoints q0b, and a region [10][n] for the
+// edge states, replaced in place by the edge-state adjoints (every slot is
+// read and written by the one face thread that owns it) and then reused,
+// one variable at a time, for the stencil adjoints.  adj_out is read from
+// global memory where it is needed.
 // ---------------------------------------------------------------------------
 template <int ORD, int NGLL>
-constexpr int adj_x_nb() {
-  return 5 * ORD > 5 * NGLL + 10 ? 5 * ORD : 5 * NGLL + 10;
+constexpr int adj_line_rows() {
+  return 5 * NGLL + 10;
 }
 
-template <int ORD, int NGLL>
-__global__ __launch_bounds__(256) void k_adj_x(const double* __restrict__ in, const double* __restrict__ adj_out,
-                                               double* __restrict__ adj_in, Grid G, SweepK K) {
+template <int ORD, int NGLL, int DIR>
+__global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in, const double* __restrict__ adj_out,
+                                                  double* __restrict__ adj_in, Grid G, SweepK K) {
+  static_assert(ORD <= 10, "stencil adjoints of one variable fit the edge region");
   constexpr int hs = (ORD - 1) / 2;
-  constexpr int Q0C = 0, EBC = 5 * NGLL;     // component offsets in the shared region
+  constexpr int NM = 1 + DIR;
   extern __shared__ double sm[];
-  const int nx = G.nx;
-  const int W = nx + 2 * hs;
+  const int n = axis_len<DIR>(G);
+  const int W = n + 2 * hs;
+  // line -> its first interior cell, the stride along DIR and the level k of
+  // an x or y line
   const unsigned line = tile_index();
-  const int j = line % G.ny;
-  const int k = line / G.ny;
-  double* s_st = sm;
-  double* s_ao = s_st + 5 * W;
-  double* s_e = s_ao + 5 * nx;
-  double* s_b = s_e + 10 * nx;
-  const int64_t row = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs;
-  const bool per = G.bc[0] == AWFL_BC_PERIODIC;
-  // 1. stage the line (boundary rule at load) and its adjoint
+  int64_t base, stride;
+  int kl = 0;
+  if constexpr (DIR == 0) {
+    const int j = line % G.ny;
+    kl = line / G.ny;
+    base = (int64_t)(hs + kl) * G.sz + (int64_t)(hs + j) * G.sy + hs;
+    stride = 1;
+  } else if constexpr (DIR == 1) {
+    const int i = line % G.nx;
+    kl = line / G.nx;
+    base = (int64_t)(hs + kl) * G.sz + (int64_t)hs * G.sy + hs + i;
+    stride = G.sy;
+  } else {
+    const int i = line % G.nx;
+    const int j = line / G.nx;
+    base = (int64_t)hs * G.sz + (int64_t)(hs + j) * G.sy + hs + i;
+    stride = G.sz;
+  }
+  double* s_st = sm;                 // [5][W]
+  double* s_q = s_st + 5 * W;        // [5 NGLL][n]: q0, then q0b
+  double* s_e = s_q + 5 * NGLL * n;  // [10][n]: eL|eR, then eLb|eRb, then stencil adjoints
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const double* ao = adj_out + base;
+  // background of cell c: the line's level for x and y, the cell's own for z
+  auto bg_of = [&](int c) { return cell_bg<NGLL, DIR>(G, DIR == 2 ? c : kl); };
+  // 1. stage the line (boundary rule at load)
   for (int c = threadIdx.x; c < W; c += blockDim.x) {
     bool zn;
-    const int src = map_axis(c - hs, nx, per, zn);
-    sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == idU) ? 0.0 : in[l * G.sl + row + src]; });
+    const int src = map_axis(c - hs, n, per, zn);
+    sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
   }
-  for (int c = threadIdx.x; c < nx; c += blockDim.x)
-    sfor<5>([&](auto l) { s_ao[l * nx + c] = adj_out[l * G.sl + row + c]; });
   __syncthreads();
-  const CellBg<NGLL> bg = cell_bg<NGLL, 0>(G, k);
-  // 2. cell forward: WENO values and edge states
-  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
-    double w[5][ORD], q0[5][NGLL], eL[5], eR[5], sa;
-    sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { w[l][s] = s_st[l * W + c + s]; }); });
-    adj::cell_recon<ORD, NGLL>(w, G.weno_s, G.weno_w, q0);
-    sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { s_b[(Q0C + l * NGLL + ii) * nx + c] = q0[l][ii]; }); });
+  // 2. cell forward: WENO values (one variable at a time) and edge states
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+#pragma unroll 1
+    for (int l = 0; l < 5; l++) {
+      double w[ORD], g[NGLL];
+      sfor<ORD>([&](auto s) { w[s] = s_st[l * W + c + s]; });
+      reconstruct<ORD, NGLL>(w, (l == idR || l == idT) ? G.weno_s : G.weno_w, g);
+      sfor<NGLL>([&](auto ii) { s_q[(l * NGLL + ii) * n + c] = g[ii]; });
+    }
+    double q0[5][NGLL], eL[5], eR[5], sa;
+    sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { q0[l][ii] = s_q[(l * NGLL + ii) * n + c]; }); });
     adj::Tape<NGLL, double> tp;
-    adj::cell_dt<NGLL, 0>(q0, bg, K, tp, eL, eR, sa);
+    adj::cell_dt<NGLL, DIR>(q0, bg_of(c), K, tp, eL, eR, sa);
     sfor<5>([&](auto l) {
-      s_e[l * nx + c] = eL[l];
-      s_e[(5 + l) * nx + c] = eR[l];
+      s_e[l * n + c] = eL[l];
+      s_e[(5 + l) * n + c] = eR[l];
     });
   }
   __syncthreads();
-  // 3. faces: flux adjoint -> edge-state adjoints (each written by one face)
+  // 3. faces: flux adjoint -> edge-state adjoints, over the edge states
   const double cdr = K.dt * K.rd;
-  const FaceRef fr = face_ref<0>(G, k, 0);
-  const int nfaces = per ? nx : nx + 1;
+  const int nfaces = per ? n : n + 1;
   for (int f = threadIdx.x; f < nfaces; f += blockDim.x) {
-    const bool wl = !per && f == 0, wr = !per && f == nx;
-    const int a = wl ? 0 : (f == 0 ? nx - 1 : f - 1);      // cell on the left
-    const int b = wr ? nx - 1 : (f == nx ? 0 : f);         // cell on the right
+    const bool wl = !per && f == 0, wr = !per && f == n;
+    const int a = wl ? 0 : (f == 0 ? n - 1 : f - 1);      // cell on the low side
+    const int b = wr ? n - 1 : (f == n ? 0 : f);          // cell on the high side
+    // slots this face owns: the low cell's high edge and the high cell's
+    // low edge (at a wall, the one edge of the boundary cell)
+    const int sL = wl ? 0 : (5 * n + a);                  // + l * n
+    const int sR = wr ? (5 * n + n - 1) : b;
     double L[5], R[5], Fb[5], Lb[5], Rb[5];
     sfor<5>([&](auto l) {
-      L[l] = wl ? s_e[l * nx + 0] : s_e[(5 + l) * nx + a];
-      R[l] = wr ? s_e[(5 + l) * nx + nx - 1] : s_e[l * nx + b];
+      L[l] = s_e[sL + l * n];
+      R[l] = s_e[sR + l * n];
       double v = 0.0;
-      if (!wr) v += s_ao[l * nx + b];
-      if (!wl) v -= s_ao[l * nx + a];
+      if (!wr) v += ao[l * G.sl + b * stride];
+      if (!wl) v -= ao[l * G.sl + a * stride];
       Fb[l] = (G.sim2d && l == idV) ? 0.0 : cdr * v;
       Lb[l] = Rb[l] = 0.0;
     });
-    adj::riemann_adj<0>(L, R, fr, Fb, Lb, Rb);
-    if (wl) {
-      sfor<5>([&](auto l) { s_b[(EBC + l) * nx + 0] = Lb[l] + Rb[l]; });
-    } else if (wr) {
-      sfor<5>([&](auto l) { s_b[(EBC + 5 + l) * nx + nx - 1] = Lb[l] + Rb[l]; });
+    adj::riemann_adj<DIR>(L, R, face_ref<DIR>(G, kl, f), Fb, Lb, Rb);
+    if (wl || wr) {
+      sfor<5>([&](auto l) { s_e[sL + l * n] = Lb[l] + Rb[l]; });
     } else {
       sfor<5>([&](auto l) {
-        s_b[(EBC + 5 + l) * nx + a] = Lb[l];
-        s_b[(EBC + l) * nx + b] = Rb[l];
+        s_e[sL + l * n] = Lb[l];
+        s_e[sR + l * n] = Rb[l];
       });
     }
   }
   __syncthreads();
-  // 4. cell reverse; the cell's stencil adjoints replace its own entries
-  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
+  // 4a. transform reverse: q0b over q0 (each cell's own column); in z the
+  // source enters rho*w as out += dt * src
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
     double q0[5][NGLL], q0b[5][NGLL], eLb[5], eRb[5];
     sfor<5>([&](auto l) {
-      sfor<NGLL>([&](auto ii) { q0[l][ii] = s_b[(Q0C + l * NGLL + ii) * nx + c]; });
-      eLb[l] = s_b[(EBC + l) * nx + c];
-      eRb[l] = s_b[(EBC + 5 + l) * nx + c];
+      sfor<NGLL>([&](auto ii) { q0[l][ii] = s_q[(l * NGLL + ii) * n + c]; });
+      eLb[l] = s_e[l * n + c];
+      eRb[l] = s_e[(5 + l) * n + c];
     });
-    adj::cell_dt_adj<NGLL, 0>(q0, bg, K, eLb, eRb, 0.0, q0b);
-    sfor<5>([&](auto l) {
-      double u[ORD], ub[ORD];
+    const double srcb = DIR == 2 ? K.dt * ao[idW * G.sl + c * stride] : 0.0;
+    adj::cell_dt_adj<NGLL, DIR>(q0, bg_of(c), K, eLb, eRb, srcb, q0b);
+    sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { s_q[(l * NGLL + ii) * n + c] = q0b[l][ii]; }); });
+  }
+  // 4b. one variable at a time: WENO reverse into the edge region (each
+  // cell's own column, whose edge adjoints step 4a has consumed), then the
+  // gather of the stencil adjoints that point at each cell
+#pragma unroll 1
+  for (int l = 0; l < 5; l++) {
+    const bool doweno = (l == idR || l == idT) ? G.weno_s : G.weno_w;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      double u[ORD], ub[ORD], gb[NGLL];
       sfor<ORD>([&](auto s) {
         u[s] = s_st[l * W + c + s];
         ub[s] = 0.0;
       });
-      adj::reconstruct_adj<ORD, NGLL>(u, (l == idR || l == idT) ? G.weno_s : G.weno_w, q0b[l], ub);
-      sfor<ORD>([&](auto s) { s_b[(l * ORD + s) * nx + c] = ub[s]; });
-    });
-  }
-  __syncthreads();
-  // 5. gather and write the line (its x halo cells are zero)
-  for (int c = threadIdx.x; c < nx; c += blockDim.x) {
-    double acc[5];
-    sfor<5>([&](auto l) { acc[l] = s_ao[l * nx + c]; });
-    sfor<ORD>([&](auto s) {
-      int src = c + hs - s;            // the cell whose stencil slot s is c
-      bool ok = true;
-      if (per) {
-        while (src < 0) src += nx;
-        while (src >= nx) src -= nx;
-      } else {
-        ok = src >= 0 && src < nx;
-      }
-      if (ok) sfor<5>([&](auto l) { acc[l] += s_b[(l * ORD + s) * nx + src]; });
-    });
-    if (!per) {
-      if (c == 0)
-        sfor<hs>([&](auto cc) {
-          sfor<ORD>([&](auto s) {
-            if constexpr (cc - hs + s < 0) {
-              if (cc < nx) sfor<5>([&](auto l) {
-                if constexpr (l != idU) acc[l] += s_b[(l * ORD + s) * nx + cc];
-              });
-            }
-          });
-        });
-      if (c == nx - 1)
-        sfor<hs>([&](auto m) {
-          const int cc = nx - 1 - m;
-          sfor<ORD>([&](auto s) {
-            if constexpr (s >= hs + 1 + m) {
-              if (cc >= 0) sfor<5>([&](auto l) {
-                if constexpr (l != idU) acc[l] += s_b[(l * ORD + s) * nx + cc];
-              });
-            }
-          });
-        });
+      sfor<NGLL>([&](auto ii) { gb[ii] = s_q[(l * NGLL + ii) * n + c]; });
+      adj::reconstruct_adj<ORD, NGLL>(u, doweno, gb, ub);
+      sfor<ORD>([&](auto s) { s_e[s * n + c] = ub[s]; });
     }
-    sfor<5>([&](auto l) { adj_in[l * G.sl + row + c] = acc[l]; });
+    __syncthreads();
+    // at a wall the stencil slots past the end read the clamped end cell;
+    // the normal momentum there is zero, so it passes nothing back
+    const bool ghost = !per && l != NM;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      double acc = ao[l * G.sl + c * stride];
+      sfor<ORD>([&](auto s) {
+        int src = c + hs - s;            // the cell whose stencil slot s is c
+        bool ok = true;
+        if (per) {
+          while (src < 0) src += n;
+          while (src >= n) src -= n;
+        } else {
+          ok = src >= 0 && src < n;
+        }
+        if (ok) acc += s_e[s * n + src];
+      });
+      if (ghost) {
+        if (c == 0)                      // slot s of cell cc reads index cc - hs + s < 0
+          sfor<hs>([&](auto cc) {
+            sfor<ORD>([&](auto s) {
+              if constexpr (cc - hs + s < 0) {
+                if (cc < n) acc += s_e[s * n + cc];
+              }
+            });
+          });
+        if (c == n - 1)                  // slot s of cell n-1-m reads index >= n
+          sfor<hs>([&](auto m) {
+            const int cc = n - 1 - m;
+            sfor<ORD>([&](auto s) {
+              if constexpr (s >= hs + 1 + m) {
+                if (cc >= 0) acc += s_e[s * n + cc];
+              }
+            });
+          });
+      }
+      adj_in[l * G.sl + base + c * stride] = acc;
+    }
+    __syncthreads();
   }
-  for (int c = threadIdx.x; c < 2 * hs; c += blockDim.x) {
-    const int64_t o = c < hs ? row - hs + c : row + nx + (c - hs);
-    sfor<5>([&](auto l) { adj_in[l * G.sl + o] = 0.0; });
+  // the halo cells of an x line (y and z halos: k_zero_halo)
+  if constexpr (DIR == 0) {
+    for (int c = threadIdx.x; c < 2 * hs; c += blockDim.x) {
+      const int64_t o = c < hs ? base - hs + c : base + n + (c - hs);
+      sfor<5>([&](auto l) { adj_in[l * G.sl + o] = 0.0; });
+    }
   }
 }
 
-// zero the y and z halo rows of a padded array (the x halo cells of interior
-// rows are written by the line kernels)
-__global__ __launch_bounds__(64) void k_zero_halo_rows(Grid G, double* __restrict__ a) {
+// zero the halo of a padded array: the y and z halo rows, and with xh also
+// the x halo cells of the interior rows (which x line kernels write)
+__global__ __launch_bounds__(64) void k_zero_halo(Grid G, bool xh, double* __restrict__ a) {
   const int ih = blockIdx.x * blockDim.x + threadIdx.x;
   if (ih >= G.nx + 2 * G.hs) return;
-  const int j = (int)blockIdx.y - G.hs, k = (int)blockIdx.z - G.hs;
-  if (j >= 0 && j < G.ny && k >= 0 && k < G.nz) return;
+  const int i = ih - G.hs, j = (int)blockIdx.y - G.hs, k = (int)blockIdx.z - G.hs;
+  if (j >= 0 && j < G.ny && k >= 0 && k < G.nz && (!xh || (i >= 0 && i < G.nx))) return;
   const int64_t t = (int64_t)blockIdx.z * G.sz + (int64_t)blockIdx.y * G.sy + ih;
   sfor<5>([&](auto l) { a[l * G.sl + t] = 0.0; });
 }
@@ -622,8 +666,8 @@ struct VariantA {
   void (*weno[3])(const double*, const double*, Grid, double*);
   void (*gather[3])(const double*, const double*, Grid, double*);
   void (*tlm[3])(const double*, const double*, const double*, const double*, Grid, SweepK, double*, double*);
-  void (*adj_x)(const double*, const double*, double*, Grid, SweepK);
-  int adj_x_nb;
+  void (*line[3])(const double*, const double*, double*, Grid, SweepK);
+  int line_rows;
 };
 
 #define AWFL_VARIANT_A(O, N)                                                                        \
@@ -634,7 +678,8 @@ struct VariantA {
         {k_dt_adj<N, 0>, k_dt_adj<N, 1>, k_dt_adj<N, 2>},                                           \
         {k_weno_adj<O, N, 0>, k_weno_adj<O, N, 1>, k_weno_adj<O, N, 2>},                            \
         {k_gather<O, 0>, k_gather<O, 1>, k_gather<O, 2>},                                           \
-        {k_tlm_update<N, 0>, k_tlm_update<N, 1>, k_tlm_update<N, 2>}, k_adj_x<O, N>, adj_x_nb<O, N>(),  \
+        {k_tlm_update<N, 0>, k_tlm_update<N, 1>, k_tlm_update<N, 2>},                               \
+        {k_adj_line<O, N, 0>, k_adj_line<O, N, 1>, k_adj_line<O, N, 2>}, adj_line_rows<O, N>(),       \
   }
 
 const VariantA kVariantsA[] = {AWFL_VARIANT_A(3, 2), AWFL_VARIANT_A(5, 3)};
@@ -645,14 +690,36 @@ const VariantA* find_variant_a(int ord, int ngll) {
   return nullptr;
 }
 
-int ensure_scratch(awfl_ctx* c) {
+int check_variant(awfl_ctx* c) {
   if (!find_variant_a(c->cfg.ord, c->cfg.ngll))
     return fail(c, AWFL_E_UNSUPPORTED, "adjoint: (ord, ngll) not compiled in");
+  return AWFL_OK;
+}
+
+// per-cell scratch of the multi-kernel sweeps (tangent; adjoint lines too
+// long for LDS)
+int ensure_cell_scratch(awfl_ctx* c) {
+  int rc = check_variant(c);
+  if (rc != AWFL_OK) return rc;
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   const int per_cell = NE + NEB + 5 * c->cfg.ord + 10 * c->cfg.ngll;
   if (!c->a_scr) HIP_TRY(c, hipMalloc(&c->a_scr, sizeof(double) * nc * per_cell));
+  return AWFL_OK;
+}
+
+// second adjoint state of the step reversal
+int ensure_tmp(awfl_ctx* c) {
+  int rc = check_variant(c);
+  if (rc != AWFL_OK) return rc;
   if (!c->a_tmp) HIP_TRY(c, hipMalloc(&c->a_tmp, sizeof(double) * c->state_elems()));
   return AWFL_OK;
+}
+
+// LDS bytes of the fused line kernel along dir (0 if it does not fit)
+size_t line_lds(const awfl_ctx* c, const VariantA* v, int dir) {
+  const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
+  const size_t b = sizeof(double) * (size_t)(5 * (n + 2 * c->hs) + v->line_rows * n);
+  return b <= 160 * 1024 ? b : 0;
 }
 
 // launch shapes: cells (ceil(nx/64), ny, nz), (cell, variable) pairs
@@ -670,27 +737,27 @@ int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* 
                               c->stream));
     return AWFL_OK;
   }
-  int rc = ensure_scratch(c);
+  int rc = check_variant(c);
   if (rc != AWFL_OK) return rc;
   const VariantA* v = find_variant_a(c->cfg.ord, c->cfg.ngll);
   const Grid G = c->grid();
   const SweepK K = sweep_constants(c, dt, dir);
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   const int ng = c->cfg.ngll;
-  if (dir == 0) {
-    const int nx = c->cfg.nx;
-    const size_t lds = sizeof(double) * (size_t)(5 * (nx + 2 * c->hs) + (15 + v->adj_x_nb) * nx);
-    if (lds <= 160 * 1024) {
-      if (lds > 64 * 1024)
-        HIP_TRY(c, hipFuncSetAttribute((const void*)v->adj_x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      const int threads = std::min(256, (nx + 63) / 64 * 64);
-      v->adj_x<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, adj_out, adj_in, G, K);
-      LAUNCH_CHECK(c);
-      k_zero_halo_rows<<<padded_grid(c), 64, 0, c->stream>>>(G, adj_in);
-      LAUNCH_CHECK(c);
-      return AWFL_OK;
-    }
+  if (const size_t lds = line_lds(c, v, dir)) {
+    const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
+    const unsigned lines = (unsigned)(nc / n);
+    if (lds > 64 * 1024)
+      HIP_TRY(c, hipFuncSetAttribute((const void*)v->line[dir], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int threads = std::min(256, (n + 63) / 64 * 64);
+    v->line[dir]<<<lines, threads, lds, c->stream>>>(in, adj_out, adj_in, G, K);
+    LAUNCH_CHECK(c);
+    k_zero_halo<<<padded_grid(c), 64, 0, c->stream>>>(G, dir != 0, adj_in);
+    LAUNCH_CHECK(c);
+    return AWFL_OK;
   }
+  rc = ensure_cell_scratch(c);
+  if (rc != AWFL_OK) return rc;
   double* E = c->a_scr;
   double* EB = E + NE * nc;
   double* STB = EB + NEB * nc;
@@ -788,7 +855,7 @@ int awfl_sweep_tangent(awfl_ctx* c, const double* d_in, const double* d_din, dou
     HIP_TRY(c, hipMemcpyAsync(d_dout, d_din, sizeof(double) * c->state_elems(), hipMemcpyDeviceToDevice, c->stream));
     return AWFL_OK;
   }
-  int rc = ensure_scratch(c);
+  int rc = ensure_cell_scratch(c);
   if (rc != AWFL_OK) return rc;
   const VariantA* v = find_variant_a(c->cfg.ord, c->cfg.ngll);
   const Grid G = c->grid();
@@ -815,7 +882,7 @@ int awfl_sweep_adjoint(awfl_ctx* c, const double* d_in, const double* d_adj_out,
 int awfl_time_step_adjoint(awfl_ctx* c, const double* d_state_before, double* d_adj, double dt, int dim_switch) {
   if (!c) return AWFL_E_ARG;
   if (!d_state_before || !d_adj) return fail(c, AWFL_E_ARG, "null buffer");
-  int rc = ensure_scratch(c);
+  int rc = ensure_tmp(c);
   if (rc != AWFL_OK) return rc;
   rc = record_steps(c, d_state_before, 0, 1, dim_switch ? 1 : 0, dt);
   if (rc != AWFL_OK) return rc;
@@ -842,7 +909,7 @@ int awfl_adjoint_ckpt(awfl_ctx* c, const double* d_ckpt, double dt, int nsteps, 
   if (!c) return AWFL_E_ARG;
   if (!d_ckpt || !d_adj) return fail(c, AWFL_E_ARG, "null buffer");
   if (nsteps < 0 || every < 1) return fail(c, AWFL_E_ARG, "nsteps >= 0 and every >= 1");
-  int rc = ensure_scratch(c);
+  int rc = ensure_tmp(c);
   if (rc != AWFL_OK) return rc;
   const int64_t el = c->state_elems();
   const int ds0 = dim_switch0 ? 1 : 0;

@@ -60,6 +60,11 @@ CASES = [
     dict(n=(24, 20, 16), bc=(0, 0, 0), ord=5, ngll=3),
     dict(n=(20, 16, 12), bc=(0, 1, 1), ord=3, ngll=2),
     dict(n=(32, 1, 16), bc=(0, 0, 1), ord=5, ngll=3),
+    # x lines longer than one work-group (threads take two cells each)
+    dict(n=(300, 6, 5), bc=(1, 0, 1), ord=5, ngll=3),
+    # lines too long for LDS: the multi-kernel adjoint (x, then z)
+    dict(n=(704, 4, 3), bc=(0, 1, 1), ord=5, ngll=3),
+    dict(n=(4, 3, 704), bc=(0, 0, 1), ord=5, ngll=3),
 ]
 
 
