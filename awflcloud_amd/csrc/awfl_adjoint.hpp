@@ -404,6 +404,30 @@ AWFL_HD void tv_adj(const double (&a)[M], double tvb, double (&ab)[M]) {
   });
 }
 
+// The bridge polynomial's coefficients as one linear map of the stencil,
+// [cell][coef]: (recon[hs+1] - sum_i idl_i recon[i] on cells i..i+hs) /
+// idl[hs+1].  The reverse uses it in place of the three statements of
+// weno_coefs (full-stencil polynomial, subtraction, scaling); its values
+// differ from theirs by rounding only.
+template <int ORD>
+struct BridgeMap {
+  double m[ORD][ORD];
+  constexpr BridgeMap() : m{} {
+    using W = Weno<ORD>;
+    constexpr int hs = W::hs;
+    for (int s = 0; s < ORD; s++)
+      for (int ii = 0; ii < ORD; ii++) {
+        double t = W::recon[hs + 1][s][ii];
+        if (ii <= hs)
+          for (int i = 0; i <= hs; i++)
+            if (s - i >= 0 && s - i <= hs) t -= W::idl[i] * W::recon[i][s - i][ii];
+        m[s][ii] = t * (1.0 / W::idl[hs + 1]);
+      }
+  }
+};
+template <int ORD>
+inline constexpr BridgeMap<ORD> kBridge{};
+
 // ub += (d reconstruct / du)^T gb: WENO (weno_coefs, then the projection to
 // GLL points) or the plain projection of the stencil
 template <int ORD, int NGLL>
@@ -437,16 +461,11 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   sfor<ORD>([&](auto ii) {
     double t = 0.0;
     sfor<ORD>([&](auto s) {
-      constexpr double cf = W::recon[hs + 1][s][ii];
+      constexpr double cf = kBridge<ORD>.m[s][ii];
       if constexpr (cf != 0) t += cf * u[s];
     });
     a[hs + 1][ii] = t;
   });
-  sfor<hs + 1>([&](auto i) {
-    sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= W::idl[i] * a[i][ii]; });
-  });
-  constexpr double ridl = 1.0 / W::idl[hs + 1];
-  sfor<ORD>([&](auto ii) { a[hs + 1][ii] *= ridl; });
   double tv[NP];
   sfor<hs + 1>([&](auto i) {
     double ai[hs + 1];
@@ -533,13 +552,7 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
     sfor<hs + 1>([&](auto ii) { ab[i][ii] += aib[ii]; });
   });
   tv_adj<ORD>(a[hs + 1], tvhib, ab[hs + 1]);
-  // bridge
-  double rawb[ORD];
-  sfor<ORD>([&](auto ii) { rawb[ii] = ab[hs + 1][ii] * ridl; });
-  sfor<hs + 1>([&](auto i) {
-    sfor<hs + 1>([&](auto ii) { ab[i][ii] -= W::idl[i] * rawb[ii]; });
-  });
-  // candidate and full-stencil polynomials
+  // candidate and bridge polynomials
   sfor<hs + 1>([&](auto i) {
     sfor<hs + 1>([&](auto ii) {
       sfor<hs + 1>([&](auto s) {
@@ -550,8 +563,8 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   });
   sfor<ORD>([&](auto ii) {
     sfor<ORD>([&](auto s) {
-      constexpr double cf = W::recon[hs + 1][s][ii];
-      if constexpr (cf != 0) ub[s] += cf * rawb[ii];
+      constexpr double cf = kBridge<ORD>.m[s][ii];
+      if constexpr (cf != 0) ub[s] += cf * ab[hs + 1][ii];
     });
   });
 }
