@@ -15,8 +15,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 # translation units of libawfl_dycore.so: FP64 sweeps + ABI, FP32 perturbation form, adjoint,
-# alternative time integrators
-HIP_UNITS = ["awfl_dycore.hip", "awfl_pert.hip", "awfl_adjoint.hip", "awfl_splitting.hip"]
+# alternative time integrators, tracers
+HIP_UNITS = ["awfl_dycore.hip", "awfl_pert.hip", "awfl_adjoint.hip", "awfl_splitting.hip", "awfl_tracers.hip"]
 
 
 def run(cmd, **kw):

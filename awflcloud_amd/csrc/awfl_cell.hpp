@@ -286,10 +286,13 @@ struct SweepK {
 // along the sweep axis -> time-averaged left/right edge states (and, for z,
 // the time- and cell-averaged gravity source of rho*w).
 // ---------------------------------------------------------------------------
-template <int ORD, int NGLL, int DIR>
+// levels(q) receives the transform levels q[var][level][point] (level 0 with
+// the background added) once they are complete: the tracer sweeps take the
+// density and normal-momentum levels from it (S:757-758).
+template <int ORD, int NGLL, int DIR, class Levels>
 __host__ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s,
                                            bool weno_w, const SweepK& K, double (&eL)[5], double (&eR)[5],
-                                           double& src_avg) {
+                                           double& src_avg, Levels&& levels) {
   constexpr int NA = NGLL;             // nAder = ngll (Temporal_ader.h:8)
   constexpr int NM = 1 + DIR;          // momentum normal to the sweep
   double q[5][NA][NGLL];               // r, ru, rv, rw, rt at every level
@@ -400,6 +403,7 @@ __host__ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], 
     }
   });
 
+  levels(q);
   // time averages of the edge values (S:2142-2177)
   sfor<5>([&](auto l) {
     double a0 = q[l][0][0], a1 = q[l][0][NGLL - 1];
@@ -424,14 +428,23 @@ __host__ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], 
   }
 }
 
+template <int ORD, int NGLL, int DIR>
+__host__ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], const CellBg<NGLL>& bg, bool weno_s,
+                                           bool weno_w, const SweepK& K, double (&eL)[5], double (&eR)[5],
+                                           double& src_avg) {
+  cell_edges<ORD, NGLL, DIR>(st, bg, weno_s, weno_w, K, eL, eR, src_avg, [](const auto&) {});
+}
+
 // ---------------------------------------------------------------------------
 // Upwind characteristic flux at one face.  X and Z average primitive
 // variables; Y averages the conservative variables (S:1210-1214).  The
 // normal velocity test is strict (u > 0).
 // ---------------------------------------------------------------------------
+// un_out receives the averaged normal velocity the flux upwinds with (the
+// tracer fluxes upwind with the same value, S:891-900).
 template <int DIR>
 __host__ __device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], const FaceRef& fr,
-                                        double (&F)[5]) {
+                                        double (&F)[5], double& un_out) {
   constexpr int NM = 1 + DIR;
   // tangential momentum indices in the order the reference uses them
   constexpr int T1 = DIR == 0 ? 2 : 1;
@@ -480,6 +493,14 @@ __host__ __device__ __forceinline__ void riemann(const double (&L)[5], const dou
   F[T1] = qn * qt1 * rq1;
   F[T2] = qn * qt2 * rq1;
   F[4] = qn * q5 * rq1;
+  un_out = un;
+}
+
+template <int DIR>
+__host__ __device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], const FaceRef& fr,
+                                        double (&F)[5]) {
+  double un;
+  riemann<DIR>(L, R, fr, F, un);
 }
 
 }  // namespace awfl

@@ -103,6 +103,10 @@ struct awfl_ctx {
   int64_t a_states_n = 0;
   // alternative integrators (awfl_splitting.hip): two tendency-sized arrays
   double* r_tend[2] = {nullptr, nullptr};
+  // tracers (awfl_tracers.hip): count, positivity bit mask, ping-pong arrays
+  int ntr = 0;
+  unsigned tpos = 0;
+  double* t_buf[2] = {nullptr, nullptr};
   int n_red = 0;
   std::string err;
   Grid grid() const;

@@ -126,6 +126,12 @@ def lib():
         L.awfl_time_step_adjoint.argtypes = [P, P, P, D, C.c_int]
         L.awfl_forward_ckpt.argtypes = [P, P, D, C.c_int, C.c_int, P]
         L.awfl_adjoint_ckpt.argtypes = [P, P, D, C.c_int, C.c_int, C.c_int, P]
+        L.awfl_set_tracers.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+        L.awfl_num_tracers.argtypes = [P]
+        L.awfl_compute_tendencies_tracers.argtypes = [P, P, P, P, P, D, C.c_int]
+        L.awfl_apply_tracer_tendencies.argtypes = [P, P, P, D]
+        L.awfl_sweep_tracers.argtypes = [P, P, P, P, P, D, C.c_int]
+        L.awfl_time_step_tracers.argtypes = [P, P, P, D]
         _lib = L
     return _lib
 
@@ -152,6 +158,7 @@ class Dycore:
         self.nx, self.ny, self.nz, self.ord, self.ngll = nx, ny, nz, ord, ngll
         self.hs = (ord - 1) // 2
         self.dx, self.dy, self.dz = xlen / nx, ylen / ny, zlen / nz
+        self.ntr = 0
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         self._check(self.L.awfl_set_stream(self.h, C.c_void_p(stream)))
@@ -248,6 +255,59 @@ class Dycore:
         for spl in range(self.num_split()):
             self.compute_tendencies(state, tend, dt, spl)
             self.apply_tendencies(state, tend, dt, spl)
+
+    # --- tracers (masses rho*q, laid out like the state, one slab per tracer) ---
+    def set_tracers(self, positive):
+        """positive: one tracerPos flag per tracer"""
+        flags = (C.c_int * max(1, len(positive)))(*[int(bool(p)) for p in positive])
+        self._check(self.L.awfl_set_tracers(self.h, len(positive), flags))
+        self.ntr = len(positive)
+
+    @property
+    def tracer_shape(self):
+        hs = self.hs
+        return (self.ntr, self.nz + 2 * hs, self.ny + 2 * hs, self.nx + 2 * hs)
+
+    @property
+    def tracer_tend_shape(self):
+        return (self.ntr, self.nz, self.ny, self.nx)
+
+    def new_tracers(self):
+        return self.torch.zeros(self.tracer_shape, dtype=self.torch.float64, device="cuda")
+
+    def _chk_tr(self, t, shape):
+        if t.dtype != self.torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
+            raise ValueError("tracer array must be a contiguous float64 device tensor of shape %s" % (shape,))
+
+    def sweep_tracers(self, state, tracers, out, tr_out, dt, direction):
+        self._chk_state(state)
+        self._chk_state(out)
+        self._chk_tr(tracers, self.tracer_shape)
+        self._chk_tr(tr_out, self.tracer_shape)
+        self._check(self.L.awfl_sweep_tracers(self.h, self._ptr(state), self._ptr(tracers), self._ptr(out),
+                                              self._ptr(tr_out), dt, direction))
+
+    def compute_tendencies_tracers(self, state, tracers, tend, ttend, dt, split):
+        self._chk_state(state)
+        self._chk_tend(tend)
+        self._chk_tr(tracers, self.tracer_shape)
+        self._chk_tr(ttend, self.tracer_tend_shape)
+        self._check(self.L.awfl_compute_tendencies_tracers(self.h, self._ptr(state), self._ptr(tracers),
+                                                           self._ptr(tend), self._ptr(ttend), dt, split))
+
+    def apply_tracer_tendencies(self, tracers, ttend, dt):
+        self._chk_tr(tracers, self.tracer_shape)
+        self._chk_tr(ttend, self.tracer_tend_shape)
+        self._check(self.L.awfl_apply_tracer_tendencies(self.h, self._ptr(tracers), self._ptr(ttend), dt))
+
+    def time_step_tracers(self, state, tracers, dt):
+        self._chk_state(state)
+        self._chk_tr(tracers, self.tracer_shape)
+        self._check(self.L.awfl_time_step_tracers(self.h, self._ptr(state), self._ptr(tracers), dt))
+
+    def tracer_interior(self, tr):
+        hs = self.hs
+        return tr[:, hs:hs + self.nz, hs:hs + self.ny, hs:hs + self.nx]
 
     def time_step_strang(self, state, dt):
         """x, y with dt and z as two dt/2 sweeps (qjrmsCode/TimeIntegrator.cpp:32-65)"""

@@ -207,6 +207,41 @@ int awfl_forward_ckpt(awfl_ctx* ctx, double* d_state, double dt, int nsteps, int
 int awfl_adjoint_ckpt(awfl_ctx* ctx, const double* d_ckpt, double dt, int nsteps, int every, int dim_switch0,
                       double* d_adj);
 
+/* ---- tracers ----
+ * The reference's TracerArr / TracerTendArr and numTracers / tracerPos
+ * (S:331-339, S:565 computeTendencies(state, stateTend, tracers, tracerTend,
+ * dt, split), Temporal_ader.h:67-69), whose kernels it compiles out with
+ * #define NO_TRACERS (S:10); driver.cpp:27-29 allocates 3 tracers.  Tracer
+ * masses rho*q are laid out like the state, [ntr][nz+2hs][ny+2hs][nx+2hs];
+ * tracer tendencies [ntr][nz][ny][nx].  FP64 state only. */
+#define AWFL_MAX_TRACERS 8
+
+/* Number of tracers (0..AWFL_MAX_TRACERS) and, per tracer, the tracerPos flag
+ * (h_positive[t] != 0): clipping of its reconstructed values at zero and the
+ * positivity flux limiter (S:904-935). */
+int awfl_set_tracers(awfl_ctx* ctx, int num_tracers, const int* h_positive);
+int awfl_num_tracers(const awfl_ctx* ctx);
+
+/* computeTendencies with tracers (S:565-594 with the tracer blocks of
+ * S:598-960 and their y/z analogues): state tendency as
+ * awfl_compute_tendencies, tracer tendency beside it; toggles dimSwitch after
+ * split 2.  d_tracers is read only (the reference divides it by the density
+ * and multiplies it back in place). */
+int awfl_compute_tendencies_tracers(awfl_ctx* ctx, const double* d_state, const double* d_tracers, double* d_tend,
+                                    double* d_tracer_tend, double dt, int split);
+
+/* tracers += dt * tracer_tend over the interior (Temporal_ader.h:67-69). */
+int awfl_apply_tracer_tendencies(awfl_ctx* ctx, double* d_tracers, const double* d_tracer_tend, double dt);
+
+/* One split sweep of state and tracers, out of place: the state output is
+ * awfl_sweep's (the same statements in another kernel; equal to rounding). */
+int awfl_sweep_tracers(awfl_ctx* ctx, const double* d_in, const double* d_tr_in, double* d_out, double* d_tr_out,
+                       double dt, int dir);
+
+/* Temporal_ader::timeStep with tracers (Temporal_ader.h:48-72): the three
+ * splits of state and tracers in place; flips dimSwitch. */
+int awfl_time_step_tracers(awfl_ctx* ctx, double* d_state, double* d_tracers, double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,10 @@ struct oracle_ctx {
   double *hyDensGLL, *hyPressureGLL, *hyThetaGLL, *hyDensThetaGLL;
   /* scratch of the unfused sweep (S:327-330) and the tendency (Temporal_ader.h:36) */
   double *stateLimits, *stateFlux, *tend;
+  /* tracers (S:331-339; numTracers, tracerPos) */
+  int ntr;
+  int tpos[OR_MAXTR];
+  double *tracerLimits, *tracerFlux, *tracerTend, *tracerFac;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -73,6 +77,11 @@ struct oracle_ctx {
   c->stateLimits[((((size_t)(l) * 2 + (s)) * (c->nz + 1) + (k)) * (c->ny + 1) + (j)) * (c->nx + 1) + (i)]
 #define SF(l, k, j, i) c->stateFlux[(((size_t)(l) * (c->nz + 1) + (k)) * (c->ny + 1) + (j)) * (c->nx + 1) + (i)]
 #define HYGLL(a, k, kk) (a)[(size_t)(k) * c->ngll + (kk)]
+#define TRC(t, k, j, i) tracers[(((size_t)(t) * NZH + (k)) * NYH + (j)) * NXH + (i)]
+#define TTD(t, k, j, i) ttend[(((size_t)(t) * c->nz + (k)) * c->ny + (j)) * c->nx + (i)]
+#define TLM(t, s, k, j, i) \
+  c->tracerLimits[((((size_t)(t) * 2 + (s)) * (c->nz + 1) + (k)) * (c->ny + 1) + (j)) * (c->nx + 1) + (i)]
+#define TFX(t, k, j, i) c->tracerFlux[(((size_t)(t) * (c->nz + 1) + (k)) * (c->ny + 1) + (j)) * (c->nx + 1) + (i)]
 
 static double dmax(double a, double b) { return a > b ? a : b; }
 static double dmin(double a, double b) { return a < b ? a : b; }
@@ -343,6 +352,7 @@ oracle_ctx* oracle_create(const oracle_config* cfg) {
 void oracle_destroy(oracle_ctx* c) {
   if (!c) return;
   free(c->stateLimits); free(c->stateFlux); free(c->tend);
+  free(c->tracerLimits); free(c->tracerFlux); free(c->tracerTend); free(c->tracerFac);
   free(c->hyDensCells); free(c->hyPressureCells); free(c->hyThetaCells); free(c->hyDensThetaCells);
   free(c->hyDensGLL); free(c->hyPressureGLL); free(c->hyThetaGLL); free(c->hyDensThetaGLL);
   free(c);
@@ -893,11 +903,211 @@ void oracle_riemann(const oracle_ctx* c, int dir, const double* qL, const double
 /* ------------------------------------------------------------------------ */
 #define OMP_FOR2 _Pragma("omp parallel for collapse(2) schedule(static) num_threads(nthr)")
 
+/* ------------------------------------------------------------------------ */
+/* Tracers.  The reference compiles every tracer block out (#define         */
+/* NO_TRACERS, S:10); this restates the source text of those blocks.  Three */
+/* departures, each where the text cannot run as written:                  */
+/*  T1 positivity limiter (S:904-935, 1262-1294, 1646-1678): every face's    */
+/*     reduction factor is formed from the unlimited fluxes, then applied   */
+/*     (two passes).  The reference scales tracerFlux in place while the    */
+/*     neighbouring faces read it, an order-dependent result.               */
+/*  T2 the y limiter's v < 0 branch reads tracerFlux(tr,k,j,ind_j) and      */
+/*     (tr,k,j,ind_j+1) (S:1283-1284), faces of the x index; here it reads  */
+/*     the y faces ind_j and ind_j+1 as its v > 0 branch and the x limiter  */
+/*     do.                                                                  */
+/*  T3 the z limiter wraps ind_k like x and y (S:1657-1666 would index      */
+/*     k = -1 and nz+1 on a periodic z axis).                               */
+/* ------------------------------------------------------------------------ */
+int oracle_set_tracers(oracle_ctx* c, int ntr, const int* pos) {
+  if (ntr < 0 || ntr > OR_MAXTR) return -1;
+  free(c->tracerLimits); free(c->tracerFlux); free(c->tracerTend); free(c->tracerFac);
+  c->tracerLimits = c->tracerFlux = c->tracerTend = c->tracerFac = 0;
+  c->ntr = ntr;
+  for (int t = 0; t < OR_MAXTR; t++) c->tpos[t] = (t < ntr && pos) ? (pos[t] != 0) : 0;
+  if (ntr == 0) return 0;
+  size_t nf = (size_t)ntr * (c->nz + 1) * (c->ny + 1) * (c->nx + 1);
+  c->tracerLimits = (double*)calloc(2 * nf, sizeof(double));
+  c->tracerFlux = (double*)calloc(nf, sizeof(double));
+  c->tracerFac = (double*)calloc(nf, sizeof(double));
+  c->tracerTend = (double*)calloc((size_t)ntr * c->nz * c->ny * c->nx, sizeof(double));
+  if (!c->tracerLimits || !c->tracerFlux || !c->tracerFac || !c->tracerTend) return -1;
+  return 0;
+}
+int oracle_num_tracers(const oracle_ctx* c) { return c->ntr; }
 
-static void tend_x(oracle_ctx* c, double* state, double* tend, double dt) {
+/* tracer mass -> mixing ratio inside the domain (S:622-627, 983-989, 1342-1349) */
+static void tracer_divide(oracle_ctx* c, const double* state, double* tracers) {
+  int hs = c->hs;
+  for (int t = 0; t < c->ntr; t++)
+    for (int k = 0; k < c->nz; k++)
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++)
+          TRC(t, hs + k, hs + j, hs + i) /= (ST(idR, hs + k, hs + j, hs + i) + c->hyDensCells[hs + k]);
+}
+
+/* mixing ratio -> tracer mass (S:952-954, 1306-1308, 1692-1694) */
+static void tracer_multiply(oracle_ctx* c, const double* state, double* tracers) {
+  int hs = c->hs;
+  for (int t = 0; t < c->ntr; t++)
+    for (int k = 0; k < c->nz; k++)
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++)
+          TRC(t, hs + k, hs + j, hs + i) *= (ST(idR, hs + k, hs + j, hs + i) + c->hyDensCells[hs + k]);
+}
+
+/* halos of the mixing ratio along dir: periodic copy, or the end cell's value
+ * at a wall (S:636-640, 655-660, 1000-1004, 1018-1022, 1357-1361, 1375-1379) */
+static void tracer_halo(oracle_ctx* c, double* tracers, int dir) {
+  int hs = c->hs, nx = c->nx, ny = c->ny, nz = c->nz;
+  int bc = dir == 0 ? c->cfg.bc_x : dir == 1 ? c->cfg.bc_y : c->cfg.bc_z;
+  for (int t = 0; t < c->ntr; t++)
+    for (int a = 0; a < (dir == 2 ? ny : nz); a++)
+      for (int b = 0; b < (dir == 0 ? ny : nx); b++)
+        for (int h = 0; h < hs; h++) {
+          if (dir == 0) {
+            int k = a, j = b;
+            TRC(t, hs + k, hs + j, h) = bc == BC_PERIODIC ? TRC(t, hs + k, hs + j, nx + h) : TRC(t, hs + k, hs + j, hs);
+            TRC(t, hs + k, hs + j, hs + nx + h) =
+                bc == BC_PERIODIC ? TRC(t, hs + k, hs + j, hs + h) : TRC(t, hs + k, hs + j, hs + nx - 1);
+          } else if (dir == 1) {
+            int k = a, i = b;
+            TRC(t, hs + k, h, hs + i) = bc == BC_PERIODIC ? TRC(t, hs + k, ny + h, hs + i) : TRC(t, hs + k, hs, hs + i);
+            TRC(t, hs + k, hs + ny + h, hs + i) =
+                bc == BC_PERIODIC ? TRC(t, hs + k, hs + h, hs + i) : TRC(t, hs + k, hs + ny - 1, hs + i);
+          } else {
+            int j = a, i = b;
+            TRC(t, h, hs + j, hs + i) = bc == BC_PERIODIC ? TRC(t, nz + h, hs + j, hs + i) : TRC(t, hs, hs + j, hs + i);
+            TRC(t, hs + nz + h, hs + j, hs + i) =
+                bc == BC_PERIODIC ? TRC(t, hs + h, hs + j, hs + i) : TRC(t, hs + nz - 1, hs + j, hs + i);
+          }
+        }
+}
+
+/* diffTransformTracer (S:2095-2126): rho*tracer rt and its flux rut from the
+ * density r and the normal momentum ru at every level */
+static void dt_tracer(const oracle_ctx* c, arr2 r, arr2 ru, arr2 rt, arr2 rut, double dx) {
+  int nAder = c->nader, ngll = c->ngll;
+  for (int kt = 1; kt < nAder; kt++)
+    for (int ii = 0; ii < ngll; ii++) rut[kt][ii] = 0;
+  for (int kt = 0; kt < nAder - 1; kt++) {
+    for (int ii = 0; ii < ngll; ii++) {
+      double df_dx = 0;
+      for (int s = 0; s < ngll; s++) df_dx += c->deriv[s][ii] * rut[kt][s];
+      rt[kt + 1][ii] = -df_dx / dx / (kt + 1.);
+    }
+    for (int ii = 0; ii < ngll; ii++) {
+      double tot = 0;
+      for (int ir = 0; ir <= kt + 1; ir++) tot += ru[ir][ii] * rt[kt + 1 - ir][ii] - r[ir][ii] * rut[kt + 1 - ir][ii];
+      rut[kt + 1][ii] = tot / r[0][ii];
+    }
+  }
+}
+
+/* Edge values of rho*tracer for one cell (S:763-808, 1126-1170, 1503-1548):
+ * stencil st of the mixing ratio, density r and normal momentum rn at every
+ * transform level of the cell. */
+static void tracer_edges(const oracle_ctx* c, const double* st, arr2 r, arr2 rn, double d, double dt, int tr,
+                         double* left, double* right) {
+  int ngll = c->ngll;
+  arr2 rt, rnt;
+  reconstruct(c, st, rt[0], c->cfg.weno_scalars);
+  for (int ii = 0; ii < ngll; ii++) rt[0][ii] *= r[0][ii];
+  if (c->tpos[tr])
+    for (int ii = 0; ii < ngll; ii++) rt[0][ii] = dmax(0., rt[0][ii]);
+  for (int ii = 0; ii < ngll; ii++) rnt[0][ii] = rt[0][ii] * rn[0][ii] / r[0][ii];
+  if (c->nader > 1) dt_tracer(c, r, rn, rt, rnt, d);
+  tavg_edges(c, rt, dt);
+  if (c->tpos[tr]) {
+    rt[0][0] = dmax(0., rt[0][0]);
+    rt[0][ngll - 1] = dmax(0., rt[0][ngll - 1]);
+  }
+  *left = rt[0][0];
+  *right = rt[0][ngll - 1];
+}
+
+/* stateLimits-style boundary values of the tracer edges (S:826-835, 1186-1195, 1567-1576) */
+static void tracer_limits_bc(oracle_ctx* c, int dir) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  int bc = dir == 0 ? c->cfg.bc_x : dir == 1 ? c->cfg.bc_y : c->cfg.bc_z;
+  for (int t = 0; t < c->ntr; t++)
+    for (int a = 0; a < (dir == 2 ? ny : nz); a++)
+      for (int b = 0; b < (dir == 0 ? ny : nx); b++) {
+        int k0, j0, i0, k1, j1, i1;   /* face 0 and face n of the line */
+        if (dir == 0) { k0 = k1 = a; j0 = j1 = b; i0 = 0; i1 = nx; }
+        else if (dir == 1) { k0 = k1 = a; i0 = i1 = b; j0 = 0; j1 = ny; }
+        else { j0 = j1 = a; i0 = i1 = b; k0 = 0; k1 = nz; }
+        if (bc == BC_PERIODIC) {
+          TLM(t, 0, k0, j0, i0) = TLM(t, 0, k1, j1, i1);
+          TLM(t, 1, k1, j1, i1) = TLM(t, 1, k0, j0, i0);
+        } else {
+          TLM(t, 0, k0, j0, i0) = TLM(t, 1, k0, j0, i0);
+          TLM(t, 1, k1, j1, i1) = TLM(t, 0, k1, j1, i1);
+        }
+      }
+}
+
+/* upwind tracer fluxes at one face (S:889-901, 1248-1259, 1632-1643): the
+ * face's mass flux times rho*tracer / rho of the upwind edge; un is the
+ * normal velocity the state flux upwinds with */
+static void tracer_face(oracle_ctx* c, int k, int j, int i, double un, double massFlux, double q1_L, double q1_R) {
+  for (int t = 0; t < c->ntr; t++) {
+    if (un > 0) TFX(t, k, j, i) = massFlux * TLM(t, 0, k, j, i) / q1_L;
+    else TFX(t, k, j, i) = massFlux * TLM(t, 1, k, j, i) / q1_R;
+  }
+}
+
+/* positivity limiter along dir (S:904-935, 1262-1294, 1646-1678; T1-T3) */
+static void tracer_limit(oracle_ctx* c, const double* state, const double* tracers, int dir, double d, double dt) {
+  const double eps = 1.e-10;
+  int hs = c->hs, nx = c->nx, ny = c->ny, nz = c->nz, NM = 1 + dir;
+  int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+  int ek = dir == 2, ej = dir == 1, ei = dir == 0;
+  for (int t = 0; t < c->ntr; t++) {
+    if (!c->tpos[t]) continue;
+    /* factor of every cell from its outflow through the unlimited fluxes */
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          double f1 = dmin(TFX(t, k, j, i), 0.);
+          double f2 = dmax(TFX(t, k + ek, j + ej, i + ei), 0.);
+          double fluxOut = dt * (f2 - f1) / d;
+          double dens = ST(idR, hs + k, hs + j, hs + i) + c->hyDensCells[hs + k];
+          c->tracerFac[(((size_t)t * (nz + 1) + k) * (ny + 1) + j) * (nx + 1) + i] =
+              dmin(1., TRC(t, hs + k, hs + j, hs + i) * dens / (fluxOut + eps));
+        }
+    /* every face takes the factor of its upwind cell */
+    for (int k = 0; k < nz + ek; k++)
+      for (int j = 0; j < ny + ej; j++)
+        for (int i = 0; i < nx + ei; i++) {
+          double u = 0.5 * (SL(NM, 0, k, j, i) + SL(NM, 1, k, j, i));
+          int f = dir == 0 ? i : dir == 1 ? j : k, a;
+          if (u > 0) a = f == 0 ? n - 1 : f - 1;
+          else if (u < 0) a = f == n ? 0 : f;
+          else continue;
+          int ka = dir == 2 ? a : k, ja = dir == 1 ? a : j, ia = dir == 0 ? a : i;
+          TFX(t, k, j, i) *= c->tracerFac[(((size_t)t * (nz + 1) + ka) * (ny + 1) + ja) * (nx + 1) + ia];
+        }
+  }
+}
+
+/* tracer tendency along dir (S:948-955, 1302-1309, 1688-1695) */
+static void tracer_tend(oracle_ctx* c, double* ttend, int dir, double d) {
+  int ek = dir == 2, ej = dir == 1, ei = dir == 0;
+  for (int t = 0; t < c->ntr; t++)
+    for (int k = 0; k < c->nz; k++)
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++)
+          TTD(t, k, j, i) = -(TFX(t, k + ek, j + ej, i + ei) - TFX(t, k, j, i)) / d;
+}
+
+
+
+static void tend_x(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend, double dt) {
   int hs = c->hs, nx = c->nx, ny = c->ny, nz = c->nz, ord = c->ord, ngll = c->ngll, nthr = nthreads_of(c);
   int ws = c->cfg.weno_scalars, wwd = c->cfg.weno_winds, bc = c->cfg.bc_x;
   double dx = c->dx;
+  const int ntr = tracers ? c->ntr : 0;
+  if (ntr) tracer_divide(c, state, tracers);   /* S:622-627 */
   /* S:631-662 */
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
@@ -914,6 +1124,7 @@ static void tend_x(oracle_ctx* c, double* state, double* tend, double dt) {
             ST(l, hs + k, hs + j, hs + nx + ii) = ST(l, hs + k, hs + j, hs + nx - 1);
           }
         }
+  if (ntr) tracer_halo(c, tracers, 0);
   /* S:665-811 */
   OMP_FOR2
   for (int k = 0; k < nz; k++)
@@ -959,6 +1170,13 @@ static void tend_x(oracle_ctx* c, double* state, double* tend, double dt) {
         SL(idV, 0, k, j, i + 1) = rv[0][ngll - 1];
         SL(idW, 0, k, j, i + 1) = rw[0][ngll - 1];
         SL(idT, 0, k, j, i + 1) = rt[0][ngll - 1];
+        for (int t = 0; t < ntr; t++) {            /* S:760-809 */
+          double tst[N9], tl, tr_;
+          for (int ii = 0; ii < ord; ii++) tst[ii] = TRC(t, hs + k, hs + j, i + ii);
+          tracer_edges(c, tst, r, ru, dx, dt, t, &tl, &tr_);
+          TLM(t, 1, k, j, i) = tl;
+          TLM(t, 0, k, j, i + 1) = tr_;
+        }
       }
   /* S:816-837 */
   for (int k = 0; k < nz; k++)
@@ -972,6 +1190,7 @@ static void tend_x(oracle_ctx* c, double* state, double* tend, double dt) {
           SL(l, 1, k, j, nx) = SL(l, 0, k, j, nx);
         }
       }
+  if (ntr) tracer_limits_bc(c, 0);
   /* S:842-902 */
   OMP_FOR2
   for (int k = 0; k < nz; k++)
@@ -981,7 +1200,13 @@ static void tend_x(oracle_ctx* c, double* state, double* tend, double dt) {
         for (int l = 0; l < 5; l++) { L[l] = SL(l, 0, k, j, i); R[l] = SL(l, 1, k, j, i); }
         riemann_x(L, R, f);
         for (int l = 0; l < 5; l++) SF(l, k, j, i) = f[l];
+        if (ntr) tracer_face(c, k, j, i, 0.5 * (L[1] / L[0] + R[1] / R[0]), f[0], L[0], R[0]);
       }
+  if (ntr) {
+    tracer_limit(c, state, tracers, 0, dx, dt);
+    tracer_tend(c, ttend, 0, dx);
+    tracer_multiply(c, state, tracers);
+  }
   /* S:940-956 */
   OMP_FOR2
   for (int k = 0; k < nz; k++)
@@ -996,10 +1221,12 @@ static void tend_x(oracle_ctx* c, double* state, double* tend, double dt) {
 /* ------------------------------------------------------------------------ */
 /* computeTendenciesY (S:961-1311)                                          */
 /* ------------------------------------------------------------------------ */
-static void tend_y(oracle_ctx* c, double* state, double* tend, double dt) {
+static void tend_y(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend, double dt) {
   int hs = c->hs, nx = c->nx, ny = c->ny, nz = c->nz, ord = c->ord, ngll = c->ngll, nthr = nthreads_of(c);
   int ws = c->cfg.weno_scalars, wwd = c->cfg.weno_winds, bc = c->cfg.bc_y;
   double dy = c->dy;
+  const int ntr = tracers ? c->ntr : 0;
+  if (ntr) tracer_divide(c, state, tracers);   /* S:983-989 */
   /* S:994-1025 */
   for (int k = 0; k < nz; k++)
     for (int i = 0; i < nx; i++)
@@ -1016,6 +1243,7 @@ static void tend_y(oracle_ctx* c, double* state, double* tend, double dt) {
             ST(l, hs + k, hs + ny + jj, hs + i) = ST(l, hs + k, hs + ny - 1, hs + i);
           }
         }
+  if (ntr) tracer_halo(c, tracers, 1);
   /* S:1028-1171 */
   OMP_FOR2
   for (int k = 0; k < nz; k++)
@@ -1061,6 +1289,13 @@ static void tend_y(oracle_ctx* c, double* state, double* tend, double dt) {
         SL(idV, 0, k, j + 1, i) = rv_tavg[ngll - 1];
         SL(idW, 0, k, j + 1, i) = rw[0][ngll - 1];
         SL(idT, 0, k, j + 1, i) = rt[0][ngll - 1];
+        for (int t = 0; t < ntr; t++) {            /* S:1123-1171 */
+          double tst[N9], tl, tr_;
+          for (int jj = 0; jj < ord; jj++) tst[jj] = TRC(t, hs + k, j + jj, hs + i);
+          tracer_edges(c, tst, r, rv, dy, dt, t, &tl, &tr_);
+          TLM(t, 1, k, j, i) = tl;
+          TLM(t, 0, k, j + 1, i) = tr_;
+        }
       }
   /* S:1176-1197 */
   for (int k = 0; k < nz; k++)
@@ -1074,6 +1309,7 @@ static void tend_y(oracle_ctx* c, double* state, double* tend, double dt) {
           SL(l, 1, k, ny, i) = SL(l, 0, k, ny, i);
         }
       }
+  if (ntr) tracer_limits_bc(c, 1);
   /* S:1202-1260 */
   OMP_FOR2
   for (int k = 0; k < nz; k++)
@@ -1083,6 +1319,7 @@ static void tend_y(oracle_ctx* c, double* state, double* tend, double dt) {
         for (int l = 0; l < 5; l++) { L[l] = SL(l, 0, k, j, i); R[l] = SL(l, 1, k, j, i); }
         riemann_y(L, R, f);
         for (int l = 0; l < 5; l++) SF(l, k, j, i) = f[l];
+        if (ntr) tracer_face(c, k, j, i, 0.5 * (L[2] + R[2]), f[0], L[0], R[0]);   /* v of S:1211 */
       }
   /* S:1298-1310 */
   OMP_FOR2
@@ -1090,15 +1327,22 @@ static void tend_y(oracle_ctx* c, double* state, double* tend, double dt) {
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++)
         for (int l = 0; l < NUMSTATE; l++) TD(l, k, j, i) = -(SF(l, k, j + 1, i) - SF(l, k, j, i)) / dy;
+  if (ntr) {
+    tracer_limit(c, state, tracers, 1, dy, dt);
+    tracer_tend(c, ttend, 1, dy);
+    tracer_multiply(c, state, tracers);
+  }
 }
 
 /* ------------------------------------------------------------------------ */
 /* computeTendenciesZ (S:1315-1697)                                         */
 /* ------------------------------------------------------------------------ */
-static void tend_z(oracle_ctx* c, double* state, double* tend, double dt) {
+static void tend_z(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend, double dt) {
   int hs = c->hs, nx = c->nx, ny = c->ny, nz = c->nz, ord = c->ord, ngll = c->ngll, nthr = nthreads_of(c);
   int ws = c->cfg.weno_scalars, wwd = c->cfg.weno_winds, bc = c->cfg.bc_z;
   double dz = c->dz;
+  const int ntr = tracers ? c->ntr : 0;
+  if (ntr) tracer_divide(c, state, tracers);   /* S:1342-1349 */
   /* S:1351-1382 */
   for (int j = 0; j < ny; j++)
     for (int i = 0; i < nx; i++)
@@ -1115,6 +1359,7 @@ static void tend_z(oracle_ctx* c, double* state, double* tend, double dt) {
             ST(l, hs + nz + kk, hs + j, hs + i) = ST(l, hs + nz - 1, hs + j, hs + i);
           }
         }
+  if (ntr) tracer_halo(c, tracers, 2);
   /* S:1385-1552 */
   OMP_FOR2
   for (int k = 0; k < nz; k++)
@@ -1169,6 +1414,13 @@ static void tend_z(oracle_ctx* c, double* state, double* tend, double dt) {
         SL(idV, 0, k + 1, j, i) = rv[0][ngll - 1];
         SL(idW, 0, k + 1, j, i) = rw_tavg[ngll - 1];
         SL(idT, 0, k + 1, j, i) = rt[0][ngll - 1];
+        for (int t = 0; t < ntr; t++) {            /* S:1500-1548 */
+          double tst[N9], tl, tr_;
+          for (int kk = 0; kk < ord; kk++) tst[kk] = TRC(t, k + kk, hs + j, hs + i);
+          tracer_edges(c, tst, r, rw, dz, dt, t, &tl, &tr_);
+          TLM(t, 1, k, j, i) = tl;
+          TLM(t, 0, k + 1, j, i) = tr_;
+        }
         double src_avg = 0;
         for (int kk = 0; kk < ngll; kk++) src_avg += src[0][kk] * c->gllWts_ngll[kk];
         TD(idR, k, j, i) = 0;
@@ -1189,6 +1441,7 @@ static void tend_z(oracle_ctx* c, double* state, double* tend, double dt) {
           SL(l, 1, nz, j, i) = SL(l, 0, nz, j, i);
         }
       }
+  if (ntr) tracer_limits_bc(c, 2);
   /* S:1583-1644 */
   OMP_FOR2
   for (int k = 0; k < nz + 1; k++)
@@ -1198,7 +1451,13 @@ static void tend_z(oracle_ctx* c, double* state, double* tend, double dt) {
         for (int l = 0; l < 5; l++) { L[l] = SL(l, 0, k, j, i); R[l] = SL(l, 1, k, j, i); }
         riemann_z(c, L, R, k, f);
         for (int l = 0; l < 5; l++) SF(l, k, j, i) = f[l];
+        if (ntr) tracer_face(c, k, j, i, 0.5 * (L[3] / L[0] + R[3] / R[0]), f[0], L[0], R[0]);
       }
+  if (ntr) {
+    tracer_limit(c, state, tracers, 2, dz, dt);
+    tracer_tend(c, ttend, 2, dz);
+    tracer_multiply(c, state, tracers);
+  }
   /* S:1680-1696 */
   OMP_FOR2
   for (int k = 0; k < nz; k++)
@@ -1210,36 +1469,55 @@ static void tend_z(oracle_ctx* c, double* state, double* tend, double dt) {
         }
 }
 
-void oracle_tendencies_dir(oracle_ctx* c, double* state, double* tend, double dt, int dir) {
-  if (dir == 0) tend_x(c, state, tend, dt);
+void oracle_tendencies_dir_tr(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend, double dt,
+                              int dir) {
+  if (dir == 0) tend_x(c, state, tend, tracers, ttend, dt);
   else if (dir == 1) {
-    if (c->sim2d) memset(tend, 0, sizeof(double) * NUMSTATE * c->nx * c->ny * c->nz);   /* S:572-575 */
-    else tend_y(c, state, tend, dt);
-  } else tend_z(c, state, tend, dt);
+    if (c->sim2d) {                                                                      /* S:572-575 */
+      memset(tend, 0, sizeof(double) * NUMSTATE * c->nx * c->ny * c->nz);
+      if (tracers && c->ntr) memset(ttend, 0, sizeof(double) * c->ntr * c->nx * c->ny * c->nz);
+    } else {
+      tend_y(c, state, tend, tracers, ttend, dt);
+    }
+  } else tend_z(c, state, tend, tracers, ttend, dt);
+}
+void oracle_tendencies_dir(oracle_ctx* c, double* state, double* tend, double dt, int dir) {
+  oracle_tendencies_dir_tr(c, state, tend, 0, 0, dt, dir);
 }
 
 /* S:565-594 */
-void oracle_compute_tendencies(oracle_ctx* c, double* state, double* tend, double dt, int split) {
+void oracle_compute_tendencies_tr(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend,
+                                  double dt, int split) {
   static const int fwd[3] = {0, 1, 2}, bwd[3] = {2, 1, 0};
   int dir = c->dimSwitch ? fwd[split] : bwd[split];
-  oracle_tendencies_dir(c, state, tend, dt, dir);
+  oracle_tendencies_dir_tr(c, state, tend, tracers, ttend, dt, dir);
   if (split == 2) c->dimSwitch = !c->dimSwitch;
 }
+void oracle_compute_tendencies(oracle_ctx* c, double* state, double* tend, double dt, int split) {
+  oracle_compute_tendencies_tr(c, state, tend, 0, 0, dt, split);
+}
 
-/* Temporal_ader::timeStep (Temporal_ader.h:48-72) */
-void oracle_time_step(oracle_ctx* c, double* state, double dt) {
+/* Temporal_ader::timeStep (Temporal_ader.h:48-72), tracers (:67-69) when given */
+void oracle_time_step_tr(oracle_ctx* c, double* state, double* tracers, double dt) {
   int hs = c->hs, nthr = nthreads_of(c);
   double* tend = c->tend;
+  double* ttend = c->tracerTend;
+  const int ntr = tracers ? c->ntr : 0;
   for (int spl = 0; spl < 3; spl++) {
     double dtloc = dt;
-    oracle_compute_tendencies(c, state, tend, dtloc, spl);
+    oracle_compute_tendencies_tr(c, state, tend, ntr ? tracers : 0, ttend, dtloc, spl);
     OMP_FOR2
     for (int l = 0; l < NUMSTATE; l++)
       for (int k = 0; k < c->nz; k++)
         for (int j = 0; j < c->ny; j++)
           for (int i = 0; i < c->nx; i++) ST(l, hs + k, hs + j, hs + i) += dtloc * TD(l, k, j, i);
+    for (int t = 0; t < ntr; t++)
+      for (int k = 0; k < c->nz; k++)
+        for (int j = 0; j < c->ny; j++)
+          for (int i = 0; i < c->nx; i++) TRC(t, hs + k, hs + j, hs + i) += dtloc * TTD(t, k, j, i);
   }
 }
+void oracle_time_step(oracle_ctx* c, double* state, double dt) { oracle_time_step_tr(c, state, 0, dt); }
 
 double oracle_run_steps(oracle_ctx* c, double* state, double dt, int nsteps) {
   struct timespec t0, t1;

@@ -22,6 +22,7 @@ extern "C" {
 #endif
 
 #define OR_MAXN 9
+#define OR_MAXTR 8
 
 typedef struct oracle_ctx oracle_ctx;
 
@@ -65,6 +66,19 @@ void oracle_tendencies_dir(oracle_ctx* c, double* state, double* tend, double dt
 void oracle_time_step(oracle_ctx* c, double* state, double dt);
 /* nsteps time steps; returns wall seconds of the stepping only. */
 double oracle_run_steps(oracle_ctx* c, double* state, double dt, int nsteps);
+
+/* Tracers (S:598-960 and the y/z analogues; compiled out in the reference by
+ * NO_TRACERS, S:10): ntr <= OR_MAXTR tracer masses rho*q laid out
+ * [ntr][nz+2hs][ny+2hs][nx+2hs], tendencies [ntr][nz][ny][nx]; pos[t] != 0
+ * marks tracerPos (clipping and the positivity flux limiter).  Returns 0. */
+int oracle_set_tracers(oracle_ctx* c, int ntr, const int* pos);
+int oracle_num_tracers(const oracle_ctx* c);
+void oracle_compute_tendencies_tr(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend,
+                                  double dt, int split);
+void oracle_tendencies_dir_tr(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend, double dt,
+                              int dir);
+/* Temporal_ader::timeStep with tracers (Temporal_ader.h:48-72). */
+void oracle_time_step_tr(oracle_ctx* c, double* state, double* tracers, double dt);
 
 int oracle_get_dim_switch(const oracle_ctx* c);
 void oracle_set_dim_switch(oracle_ctx* c, int v);

@@ -63,6 +63,11 @@ def lib():
         L.oracle_diff_transform.argtypes = [P, C.c_int, D, C.c_double, C.c_int]
         L.oracle_riemann.argtypes = [P, C.c_int, D, D, C.c_int, D]
         L.oracle_init_state.argtypes = [P, C.POINTER(OracleInitSpec), D]
+        L.oracle_set_tracers.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+        L.oracle_num_tracers.argtypes = [P]
+        L.oracle_compute_tendencies_tr.argtypes = [P, D, D, D, D, C.c_double, C.c_int]
+        L.oracle_tendencies_dir_tr.argtypes = [P, D, D, D, D, C.c_double, C.c_int]
+        L.oracle_time_step_tr.argtypes = [P, D, D, C.c_double]
         L.oracle_set_pow_jitter.argtypes = [C.c_uint64]
         L.oracle_uniform_pm1.restype = C.c_double
         L.oracle_uniform_pm1.argtypes = [C.c_uint64, C.c_uint64]
@@ -169,6 +174,48 @@ class Oracle:
 
     def time_step(self, state, dt):
         self.L.oracle_time_step(self.h, _dp(state), dt)
+
+    # --- tracers ---
+    def set_tracers(self, positive):
+        """positive: one flag per tracer (tracerPos)."""
+        flags = (C.c_int * max(1, len(positive)))(*[int(bool(p)) for p in positive])
+        if self.L.oracle_set_tracers(self.h, len(positive), flags) != 0:
+            raise ValueError("oracle_set_tracers failed")
+        self.ntr = len(positive)
+
+    def tracer_shape(self):
+        hs = self.hs
+        return (self.ntr, self.nz + 2 * hs, self.ny + 2 * hs, self.nx + 2 * hs)
+
+    def tracer_tend_shape(self):
+        return (self.ntr, self.nz, self.ny, self.nx)
+
+    def tendencies_dir_tr(self, state, tracers, dt, direction):
+        """(state tendency, tracer tendency) of one direction; tracers is
+        divided by and multiplied back by the density as the reference does."""
+        tend = np.zeros(self.tend_shape())
+        ttend = np.zeros(self.tracer_tend_shape())
+        self.L.oracle_tendencies_dir_tr(self.h, _dp(state), _dp(tend), _dp(tracers), _dp(ttend), dt, direction)
+        return tend, ttend
+
+    def compute_tendencies_tr(self, state, tracers, dt, split):
+        tend = np.zeros(self.tend_shape())
+        ttend = np.zeros(self.tracer_tend_shape())
+        self.L.oracle_compute_tendencies_tr(self.h, _dp(state), _dp(tend), _dp(tracers), _dp(ttend), dt, split)
+        return tend, ttend
+
+    def time_step_tr(self, state, tracers, dt):
+        self.L.oracle_time_step_tr(self.h, _dp(state), _dp(tracers), dt)
+
+    def tracer_interior(self, tr):
+        hs = self.hs
+        return tr[:, hs:hs + self.nz, hs:hs + self.ny, hs:hs + self.nx]
+
+    def density(self, state):
+        """Full density of every interior cell (rho' + hyDensCells)."""
+        hs = self.hs
+        rb = self.background(0)[hs:hs + self.nz]
+        return self.interior(state)[0] + rb[:, None, None]
 
     # --- alternative time integrators over the same spatial operator ---
     def sweep(self, state, dt, direction):
