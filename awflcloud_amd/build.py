@@ -64,6 +64,13 @@ def build_adapter_test():
          "-I" + os.path.join(ROOT, "include"), src, "-o", exe, "-L" + PKG, "-lawfl_dycore",
          "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../awflcloud_amd",
          "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    # the same adapter with tracer transport compiled in
+    src = os.path.join(ROOT, "tests", "native", "adapter_tracers_test.cpp")
+    exe = os.path.join(ROOT, "tests", "native", "adapter_tracers_test")
+    run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-DAWFL_TRACERS", "-I/opt/rocm/include",
+         "-I" + os.path.join(ROOT, "include"), src, "-o", exe, "-L" + PKG, "-lawfl_dycore",
+         "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../awflcloud_amd",
+         "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
 
 
 def build_oracle():

@@ -28,6 +28,7 @@
 
 #include "../../include/awfl_dycore.h"
 #include "awfl_cell.hpp"
+#include "awfl_profiles.hpp"
 #include "awfl_internal.hpp"
 
 using namespace awfl;
@@ -420,36 +421,9 @@ __global__ void k_sum_final(const double* __restrict__ partial, int nb, double* 
 }
 
 // ---------------------------------------------------------------------------
-// initial data: Profiles.h:8-73 and the ord^3 GLL quadrature of S:428-485
+// initial data: Profiles.h:8-73 (awfl_profiles.hpp) and the ord^3 GLL
+// quadrature of S:428-485
 // ---------------------------------------------------------------------------
-constexpr double CP = 1004.;
-constexpr double RD = 287.;
-constexpr double P0 = 1.0e5;
-
-__host__ __device__ inline double const_theta_density(double t0, double z) {
-  double exner = 1. - GRAV * z / (CP * t0);
-  double p = pow(exner, CP / RD) * P0;
-  double rt = pow(p / C0, 1. / GAMMA);
-  return rt / t0;
-}
-__host__ __device__ inline double const_theta_pressure(double t0, double z) {
-  double r = const_theta_density(t0, z);
-  return C0 * pow(r * t0, GAMMA);
-}
-__host__ __device__ inline double bvf_theta(double t0, double bvf, double z) { return t0 * exp(bvf * bvf * z / GRAV); }
-__host__ __device__ inline double bvf_density(double t0, double bvf, double z) {
-  double t = bvf_theta(t0, bvf, z);
-  double exner = 1. - GRAV * GRAV / (CP * bvf * bvf) * (t - t0) / (t * t0);
-  double p = pow(exner, CP / RD) * P0;
-  double rt = pow(p / C0, 1. / GAMMA);
-  return rt / t;
-}
-__host__ __device__ inline double bvf_pressure(double t0, double bvf, double z) {
-  double t = bvf_theta(t0, bvf, z);
-  double r = bvf_density(t0, bvf, z);
-  return C0 * pow(r * t, GAMMA);
-}
-
 __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -41,6 +41,7 @@
 
 #include "awfl_cell.hpp"
 #include "awfl_internal.hpp"
+#include "awfl_profiles.hpp"
 
 using namespace awfl;
 
@@ -325,6 +326,42 @@ __global__ void k_apply_tr(double* __restrict__ tr, const double* __restrict__ t
   }
 }
 
+// initTracers (S:505-560): tracer 0 uniform, 1 a replica of the thermal's
+// theta bubble, 2 a block, each times the constant-theta (300 K) density and
+// integrated with the ord^3 GLL quadrature; later tracers zero
+template <int ORD>
+__global__ void k_init_tr(double* __restrict__ tr, Grid G, int ntr, double xlen, double ylen) {
+  constexpr int hs = (ORD - 1) / 2;
+  const int64_t n = (int64_t)G.nz * G.ny * G.nx;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = t % G.nx;
+    const int64_t r = t / G.nx;
+    const int j = r % G.ny, k = r / G.ny;
+    const int64_t o = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs + i;
+    bool inside = k >= 1 * G.nz / 10 && k < 3 * G.nz / 10 && i >= 4 * G.nx / 10 && i < 6 * G.nx / 10;
+    if (!G.sim2d) inside = inside && j >= 4 * G.ny / 10 && j < 6 * G.ny / 10;
+    for (int l = 0; l < ntr; l++) {
+      double acc = 0;
+      if (l < 3) {
+        for (int kk = 0; kk < ORD; kk++)
+          for (int jj = 0; jj < ORD; jj++)
+            for (int ii = 0; ii < ORD; ii++) {
+              const double zloc = (k + 0.5) * G.dz + Gll<ORD>::pts[kk] * G.dz;
+              const double yloc = G.sim2d ? ylen / 2 : (j + 0.5) * G.dy + Gll<ORD>::pts[jj] * G.dy;
+              const double xloc = (i + 0.5) * G.dx + Gll<ORD>::pts[ii] * G.dx;
+              const double wt = Gll<ORD>::wts[kk] * Gll<ORD>::wts[jj] * Gll<ORD>::wts[ii];
+              const double rh = const_theta_density(300., zloc);
+              double tval = 1;
+              if (l == 1) tval = ellipsoid_linear(xloc, yloc, zloc, xlen / 2, ylen / 2, 2000, 2000, 2000, 2000, 2);
+              else if (l == 2) tval = inside ? 1 : 0;
+              acc += rh * tval * wt;
+            }
+      }
+      tr[l * G.sl + o] = acc;
+    }
+  }
+}
+
 __global__ void k_fill_tr(double* __restrict__ p, int64_t n, double v) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) p[e] = v;
 }
@@ -334,12 +371,14 @@ using KernelTr = void (*)(const double*, const double*, double*, double*, Grid, 
 struct VariantTr {
   int ord, ngll;
   KernelTr upd[3], tend[3];
+  void (*init)(double*, Grid, int, double, double);
 };
 
 #define AWFL_VARIANT_TR(O, N)                                                                  \
   VariantTr {                                                                                  \
     O, N, {k_sweep_tr<O, N, 0, false>, k_sweep_tr<O, N, 1, false>, k_sweep_tr<O, N, 2, false>}, \
         {k_sweep_tr<O, N, 0, true>, k_sweep_tr<O, N, 1, true>, k_sweep_tr<O, N, 2, true>},     \
+        k_init_tr<O>,                                                                          \
   }
 
 const VariantTr kVariantsTr[] = {AWFL_VARIANT_TR(3, 2), AWFL_VARIANT_TR(5, 3), AWFL_VARIANT_TR(9, 5)};
@@ -414,6 +453,22 @@ int awfl_set_tracers(awfl_ctx* c, int num_tracers, const int* h_positive) {
 }
 
 int awfl_num_tracers(const awfl_ctx* c) { return c ? c->ntr : AWFL_E_ARG; }
+
+int awfl_init_tracers(awfl_ctx* c, double* d_tracers) {
+  if (!c) return AWFL_E_ARG;
+  int rc = need_tracers(c);
+  if (rc != AWFL_OK) return rc;
+  if (!d_tracers) return fail(c, AWFL_E_ARG, "null buffer");
+  const VariantTr* v = find_variant_tr(c->cfg.ord, c->cfg.ngll);
+  if (!v) return fail(c, AWFL_E_UNSUPPORTED, "tracers: (ord, ngll) not compiled in");
+  rc = fill_tr(c, d_tracers, tracer_elems(c));   // halos zero
+  if (rc != AWFL_OK) return rc;
+  const int64_t n = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  const int blocks = (int)std::min<int64_t>((n + 127) / 128, 65536);
+  v->init<<<blocks, 128, 0, c->stream>>>(d_tracers, c->grid(), c->ntr, c->cfg.xlen, c->cfg.ylen);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
 
 int awfl_sweep_tracers(awfl_ctx* c, const double* d_in, const double* d_tr_in, double* d_out, double* d_tr_out,
                        double dt, int dir) {

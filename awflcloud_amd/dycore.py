@@ -128,6 +128,7 @@ def lib():
         L.awfl_adjoint_ckpt.argtypes = [P, P, D, C.c_int, C.c_int, C.c_int, P]
         L.awfl_set_tracers.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
         L.awfl_num_tracers.argtypes = [P]
+        L.awfl_init_tracers.argtypes = [P, P]
         L.awfl_compute_tendencies_tracers.argtypes = [P, P, P, P, P, D, C.c_int]
         L.awfl_apply_tracer_tendencies.argtypes = [P, P, P, D]
         L.awfl_sweep_tracers.argtypes = [P, P, P, P, P, D, C.c_int]
@@ -274,6 +275,12 @@ class Dycore:
 
     def new_tracers(self):
         return self.torch.zeros(self.tracer_shape, dtype=self.torch.float64, device="cuda")
+
+    def init_tracers(self):
+        """the reference's initTracers fields (uniform, theta replica, block)"""
+        tr = self.new_tracers()
+        self._check(self.L.awfl_init_tracers(self.h, self._ptr(tr)))
+        return tr
 
     def _chk_tr(self, t, shape):
         if t.dtype != self.torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:

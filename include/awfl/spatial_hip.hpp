@@ -15,6 +15,14 @@
 // Arrays are device buffers (RAII, hipMalloc).  Errors throw std::runtime_error
 // after printing, as endrun() does (const.h:76-79).
 //
+// Tracers: the reference compiles its tracer transport out (#define
+// NO_TRACERS, S:10) while its driver still adds three tracers
+// (driver.cpp:27-29).  By default this adapter does the same with
+// numTracers = 0; built with -DAWFL_TRACERS it transports the tracers added
+// by addTracer (awfl_*_tracers of awfl_dycore.h), initialises them as
+// initTracers does (S:505-560) and writes their mixing ratios to the output
+// (S:1781-1787).
+//
 // Build a program against it with any C++17 compiler:
 //   g++ -std=c++17 -D__HIP_PLATFORM_AMD__ -I/opt/rocm/include -Iinclude app.cpp \
 //       -Lawflcloud_amd -lawfl_dycore -L/opt/rocm/lib -lamdhip64
@@ -164,8 +172,9 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
   int bc_x = 0, bc_y = 0, bc_z = 0, dataSpec = 0;
   bool weno_scalars = true, weno_winds = true, sim2d = false, perturb = false;
   std::string outFile;
-  int numTracers = 0;   // tracer transport is compiled out of the reference step (S:10)
-  std::vector<std::string> tracerName;
+  int numTracers = 0;   // 0 unless built with AWFL_TRACERS (the reference's NO_TRACERS, S:10)
+  std::vector<std::string> tracerName, tracerDesc;
+  std::vector<int> tracerPos;
 
   Spatial_euler3d_cons_expl_cart_fv_Agrid() = default;
   Spatial_euler3d_cons_expl_cart_fv_Agrid(const Spatial_euler3d_cons_expl_cart_fv_Agrid&) = delete;
@@ -173,9 +182,13 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
     if (ctx_) awfl_destroy(ctx_);
   }
 
-  // S:121-125.  Names are kept; with NO_TRACERS (S:10) tracers are not
-  // transported, so the arrays are created with numTracers = 0.
-  void addTracer(std::string name, bool = true, std::string = "") { tracerName.push_back(name); }
+  // S:121-125.  Without AWFL_TRACERS the names are kept and the arrays are
+  // created with numTracers = 0.
+  void addTracer(std::string name, bool posDef = true, std::string desc = "") {
+    tracerName.push_back(name);
+    tracerDesc.push_back(desc);
+    tracerPos.push_back(posDef ? 1 : 0);
+  }
 
   // S:192-340
   void init(std::string inFile) {
@@ -202,6 +215,10 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
     cfg.weno_scalars = weno_scalars; cfg.weno_winds = weno_winds;
     cfg.ord = ord; cfg.ngll = ngll; cfg.data_spec = dataSpec; cfg.precision = AWFL_FP64;
     if (awfl_create(&cfg, &ctx_) != AWFL_OK) endrun(std::string("awfl_create: ") + awfl_last_error(nullptr));
+#ifdef AWFL_TRACERS
+    numTracers = (int)tracerName.size();   // S:318
+    if (numTracers > 0) check(awfl_set_tracers(ctx_, numTracers, tracerPos.data()));
+#endif
   }
 
   StateArr createStateArr() const { return StateArr("stateArr", numState, nz + 2 * hs, ny + 2 * hs, nx + 2 * hs); }
@@ -239,13 +256,30 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
     }
     check(awfl_init_state(ctx_, state.data(), &s));
   }
-  void initTracers(TracerArr&) {}
+  // S:505-560
+  void initTracers(TracerArr& tracers) {
+    if (numTracers > 0) check(awfl_init_tracers(ctx_, tracers.data()));
+  }
 
-  // S:565-594
-  void computeTendencies(StateArr& state, StateTendArr& tend, TracerArr&, TracerTendArr&, real& dt, int split) {
-    check(awfl_compute_tendencies(ctx_, state.data(), tend.data(), dt, split));
+  // S:565-594 (with the tracer blocks when tracers are transported)
+  void computeTendencies(StateArr& state, StateTendArr& tend, TracerArr& tracers, TracerTendArr& ttend, real& dt,
+                         int split) {
+    if (numTracers > 0)
+      check(awfl_compute_tendencies_tracers(ctx_, state.data(), tracers.data(), tend.data(), ttend.data(), dt, split));
+    else
+      check(awfl_compute_tendencies(ctx_, state.data(), tend.data(), dt, split));
     bound_state_ = &state;
     bound_tend_ = &tend;
+    bound_tracers_ = &tracers;
+    bound_ttend_ = &ttend;
+  }
+
+  // tracers += dt * tracerTend of the last computeTendencies call
+  // (Temporal_ader.h:67-69)
+  void applyTracerTendencies(real dt) {
+    if (numTracers == 0) return;
+    if (!bound_tracers_ || !bound_ttend_) endrun("applyTracerTendencies called before computeTendencies");
+    check(awfl_apply_tracer_tendencies(ctx_, bound_tracers_->data(), bound_ttend_->data(), dt));
   }
 
   // README.md:60-103 contract: apply the tendency of the last computeTendencies
@@ -282,8 +316,9 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
   // file outFile (created when etime == 0): coordinates x, y, z, background
   // hyDens/hyPressure/hyTheta, the unlimited t, and dens_pert, u, v, w,
   // pot_temp_pert, pressure_pert on (t, z, y, x), computed on the device
-  // (awfl_diagnostics); classic 64-bit-offset format (netcdf_classic.hpp)
-  void output(StateArr const& state, TracerArr const&, real etime) {
+  // (awfl_diagnostics), then tracer_<name> = tracer / rho per tracer
+  // (S:1781-1787); classic 64-bit-offset format (netcdf_classic.hpp)
+  void output(StateArr const& state, TracerArr const& tracers, real etime) {
     static const char* names[6] = {"dens_pert", "u", "v", "w", "pot_temp_pert", "pressure_pert"};
     const size_t n = (size_t)nx * ny * nz;
     double* d_out = nullptr;
@@ -295,6 +330,20 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
     (void)hipFree(d_out);
     check(rc);
     if (e != hipSuccess) endrun("output: copy to host failed");
+    std::vector<double> hq;            // mixing ratios, [numTracers][nz][ny][nx]
+    if (numTracers > 0) {
+      std::vector<double> ht = tracers.createHostCopy(), hd(nz + 2 * hs);
+      check(awfl_get_background(ctx_, 0, hd.data(), (int64_t)hd.size()));
+      hq.resize((size_t)numTracers * n);
+      for (int t = 0; t < numTracers; t++)
+        for (int k = 0; k < nz; k++)
+          for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+              const size_t c = ((size_t)k * ny + j) * nx + i;
+              const double r = h[c] + hd[hs + k];
+              hq[t * n + c] = ht[(((size_t)t * (nz + 2 * hs) + hs + k) * (ny + 2 * hs) + hs + j) * (nx + 2 * hs) + hs + i] / r;
+            }
+    }
     nc::ClassicFile f;
     int64_t rec = 0;
     if (etime == 0) {
@@ -302,6 +351,7 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
       std::vector<nc::Var> vars = {{"x", {0}}, {"y", {1}}, {"z", {2}}, {"hyDens", {2}}, {"hyPressure", {2}},
                                    {"hyTheta", {2}}, {"t", {3}}};
       for (const char* v : names) vars.push_back({v, {3, 2, 1, 0}});
+      for (int t = 0; t < numTracers; t++) vars.push_back({"tracer_" + tracerName[t], {3, 2, 1, 0}});
       f.create(outFile, dims, vars);
       std::vector<double> c(std::max(nx, std::max(ny, nz)));
       for (int i = 0; i < nx; i++) c[i] = (i + 0.5) * dx;
@@ -324,6 +374,7 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
     double t = etime;
     f.write_record("t", rec, &t);
     for (int v = 0; v < 6; v++) f.write_record(names[v], rec, h.data() + v * n);
+    for (int t = 0; t < numTracers; t++) f.write_record("tracer_" + tracerName[t], rec, hq.data() + t * n);
   }
 
   void finalize(StateArr const&, TracerArr const&) {}
@@ -349,6 +400,8 @@ class Spatial_euler3d_cons_expl_cart_fv_Agrid {
   awfl_ctx* ctx_ = nullptr;
   StateArr* bound_state_ = nullptr;
   StateTendArr* bound_tend_ = nullptr;
+  TracerArr* bound_tracers_ = nullptr;
+  TracerTendArr* bound_ttend_ = nullptr;
   std::vector<double> host_state_, host_tend_;
 };
 
@@ -382,13 +435,21 @@ class Temporal_ader {
       real dtloc = dt;
       spaceOp.computeTendencies(stateArr, stateTendArr, tracerArr, tracerTendArr, dtloc, spl);
       spaceOp.applyTendencies(ApplySingleTendency{dtloc}, spl);
+      spaceOp.applyTracerTendencies(dtloc);
     }
   }
 
   // all three splits as fused sweeps (one kernel per split, in place)
   void timeStepFused(StateArr& stateArr, real dt) {
+    if (spaceOp.numTracers > 0) endrun("timeStepFused: tracers are transported; pass them");
     if (awfl_time_step(spaceOp.context(), stateArr.data(), dt) < 0)
       endrun(std::string("awfl_time_step: ") + awfl_last_error(spaceOp.context()));
+  }
+  void timeStepFused(StateArr& stateArr, TracerArr& tracerArr, real dt) {
+    const int rc = spaceOp.numTracers > 0
+                       ? awfl_time_step_tracers(spaceOp.context(), stateArr.data(), tracerArr.data(), dt)
+                       : awfl_time_step(spaceOp.context(), stateArr.data(), dt);
+    if (rc < 0) endrun(std::string("time step: ") + awfl_last_error(spaceOp.context()));
   }
 
   void finalize(StateArr& state, TracerArr& tracers) { spaceOp.finalize(state, tracers); }
@@ -410,6 +471,7 @@ class Temporal_alt {
 
   void init(std::string inFile) { spaceOp.init(inFile); }
   void timeStep(StateArr& stateArr, TracerArr&, real dt) {
+    if (spaceOp.numTracers > 0) endrun("this integrator does not transport tracers (Temporal_ader does)");
     if (STEP(spaceOp.context(), stateArr.data(), dt) < 0)
       endrun(std::string("time step: ") + awfl_last_error(spaceOp.context()));
   }

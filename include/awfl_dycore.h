@@ -222,6 +222,12 @@ int awfl_adjoint_ckpt(awfl_ctx* ctx, const double* d_ckpt, double dt, int nsteps
 int awfl_set_tracers(awfl_ctx* ctx, int num_tracers, const int* h_positive);
 int awfl_num_tracers(const awfl_ctx* ctx);
 
+/* Spatial::initTracers (S:505-560): tracer 0 uniform, 1 a replica of the
+ * thermal's theta bubble, 2 a block in the domain centre (driver.cpp:27-29
+ * names them), each times the constant-theta density; later tracers zero,
+ * halos zero. */
+int awfl_init_tracers(awfl_ctx* ctx, double* d_tracers);
+
 /* computeTendencies with tracers (S:565-594 with the tracer blocks of
  * S:598-960 and their y/z analogues): state tendency as
  * awfl_compute_tendencies, tracer tendency beside it; toggles dimSwitch after

@@ -456,6 +456,44 @@ double oracle_uniform_pm1(uint64_t seed, uint64_t idx) {
   return (double)(h >> 11) * 0x1.0p-52 - 1.0;
 }
 
+/* initTracers (S:505-560): tracer 0 uniform (rh * 1), 1 a replica of the
+ * thermal's theta bubble (ellipsoid_linear, amplitude 2, at (xlen/2, ylen/2,
+ * 2000) with radii 2000), 2 a block (k in [nz/10, 3nz/10), i and, in 3-D, j
+ * in [4n/10, 6n/10)); every value times the constant-theta (300 K) density
+ * rh, integrated with the ord^3 GLL quadrature.  Tracers beyond the third
+ * stay zero. */
+void oracle_init_tracers(const oracle_ctx* c, double* tracers) {
+  int hs = c->hs, ord = c->ord, nx = c->nx, ny = c->ny, nz = c->nz;
+  double dx = c->dx, dy = c->dy, dz = c->dz, xlen = c->cfg.xlen, ylen = c->cfg.ylen;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        for (int l = 0; l < c->ntr; l++) {
+          TRC(l, hs + k, hs + j, hs + i) = 0;
+          for (int kk = 0; kk < ord; kk++)
+            for (int jj = 0; jj < ord; jj++)
+              for (int ii = 0; ii < ord; ii++) {
+                double zloc = (k + 0.5) * dz + c->gllPts_ord[kk] * dz;
+                double yloc = c->sim2d ? ylen / 2 : (j + 0.5) * dy + c->gllPts_ord[jj] * dy;
+                double xloc = (i + 0.5) * dx + c->gllPts_ord[ii] * dx;
+                double wt = c->gllWts_ord[kk] * c->gllWts_ord[jj] * c->gllWts_ord[ii];
+                double th = 300;
+                double rh = initConstTheta_density(th, zloc);
+                if (l == 0) {
+                  TRC(l, hs + k, hs + j, hs + i) += rh * 1 * wt;
+                } else if (l == 1) {
+                  double tval = ellipsoid_linear(xloc, yloc, zloc, xlen / 2, ylen / 2, 2000, 2000, 2000, 2000, 2);
+                  TRC(l, hs + k, hs + j, hs + i) += rh * tval * wt;
+                } else if (l == 2) {
+                  int inside = k >= 1 * nz / 10 && k < 3 * nz / 10 && i >= 4 * nx / 10 && i < 6 * nx / 10;
+                  if (!c->sim2d) inside = inside && j >= 4 * ny / 10 && j < 6 * ny / 10;
+                  double tval = inside ? 1 : 0;
+                  TRC(l, hs + k, hs + j, hs + i) += rh * tval * wt;
+                }
+              }
+        }
+}
+
 /* S:428-485 with a general list of ellipsoids, per-cell noise and a
  * band-limited field (the synthetic inputs of the benchmarks). */
 void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* sp, double* state) {

@@ -77,6 +77,9 @@ void oracle_compute_tendencies_tr(oracle_ctx* c, double* state, double* tend, do
                                   double dt, int split);
 void oracle_tendencies_dir_tr(oracle_ctx* c, double* state, double* tend, double* tracers, double* ttend, double dt,
                               int dir);
+/* initTracers (S:505-560): the reference's three tracers (uniform, theta
+ * replica, block) times the constant-theta density; later tracers zero. */
+void oracle_init_tracers(const oracle_ctx* c, double* tracers);
 /* Temporal_ader::timeStep with tracers (Temporal_ader.h:48-72). */
 void oracle_time_step_tr(oracle_ctx* c, double* state, double* tracers, double dt);
 

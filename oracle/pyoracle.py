@@ -68,6 +68,7 @@ def lib():
         L.oracle_compute_tendencies_tr.argtypes = [P, D, D, D, D, C.c_double, C.c_int]
         L.oracle_tendencies_dir_tr.argtypes = [P, D, D, D, D, C.c_double, C.c_int]
         L.oracle_time_step_tr.argtypes = [P, D, D, C.c_double]
+        L.oracle_init_tracers.argtypes = [P, D]
         L.oracle_set_pow_jitter.argtypes = [C.c_uint64]
         L.oracle_uniform_pm1.restype = C.c_double
         L.oracle_uniform_pm1.argtypes = [C.c_uint64, C.c_uint64]
@@ -203,6 +204,11 @@ class Oracle:
         ttend = np.zeros(self.tracer_tend_shape())
         self.L.oracle_compute_tendencies_tr(self.h, _dp(state), _dp(tend), _dp(tracers), _dp(ttend), dt, split)
         return tend, ttend
+
+    def init_tracers(self):
+        tr = np.zeros(self.tracer_shape())
+        self.L.oracle_init_tracers(self.h, _dp(tr))
+        return tr
 
     def time_step_tr(self, state, tracers, dt):
         self.L.oracle_time_step_tr(self.h, _dp(state), _dp(tracers), dt)

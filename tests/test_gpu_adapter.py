@@ -55,3 +55,30 @@ def test_driver_runs(tmp_path, fused):
         th0 = f.variables["pot_temp_pert"][0]
         assert 1.5 < th0.max() <= 2.0 + 2e-3 and abs(th0.min()) <= 2e-3
         assert np.abs(f.variables["w"][0]).max() == 0.0 and np.abs(f.variables["w"][2]).max() > 0.0
+
+
+def test_adapter_with_tracers(tmp_path):
+    """Adapter built with AWFL_TRACERS: the driver's three tracers through
+    Temporal_ader::timeStep and the fused step, and their mixing ratios in
+    the netCDF output (S:1781-1787)."""
+    _gpu()
+    exe = os.path.join(ROOT, "tests", "native", "adapter_tracers_test")
+    if not os.path.exists(exe):
+        pytest.fail("adapter_tracers_test not built (python -m awflcloud_amd.build)")
+    r = subprocess.run([exe, os.path.join(ROOT, "tests", "native", "adapter_small.yaml")], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    from scipy.io import netcdf_file
+    import numpy as np
+    with netcdf_file(str(tmp_path / "adapter_small.nc"), "r", mmap=False) as f:
+        for name in ("tracer_uniform", "tracer_theta", "tracer_block"):
+            assert f.variables[name].shape == (2, 16, 24, 32)
+        q = f.variables["tracer_uniform"][:].copy()
+        # rh * 1 over the constant-theta background: a mixing ratio of 1 that
+        # transport keeps
+        assert np.abs(q - 1.0).max() <= 1e-12
+        b = f.variables["tracer_block"][0]
+        assert b.max() == pytest.approx(1.0, abs=1e-12) and b.min() == 0.0
+        th = f.variables["tracer_theta"][0]
+        assert 1.0 < th.max() <= 2.0

@@ -197,3 +197,26 @@ def test_tracer_arguments_are_checked():
         D._check(D.L.awfl_time_step_tracers(D.h, D._ptr(st), D._ptr(st), 1.0))
     with pytest.raises(AwflError):
         D.set_tracers([0] * 9)
+
+
+@pytest.mark.parametrize("n,ord,ngll", [((24, 20, 12), 5, 3), ((32, 1, 16), 5, 3), ((20, 16, 10), 3, 2)])
+def test_init_tracers_match_oracle(n, ord, ngll):
+    """initTracers (S:505-560): uniform, theta replica and block fields, a
+    fourth tracer zero"""
+    _need_gpu()
+    from awflcloud_amd import Dycore
+    D = Dycore(*n, ord=ord, ngll=ngll)
+    D.set_tracers([1, 1, 1, 0])
+    g = D.init_tracers().cpu().numpy()
+    O = P.Oracle(*n, ord=ord, ngll=ngll)
+    O.set_tracers([1, 1, 1, 0])
+    o = O.init_tracers()
+    for t in range(3):
+        err = float(np.abs(g[t] - o[t]).max() / np.abs(o[t]).max())
+        print("tracer %d init err %.2e" % (t, err), flush=True)
+        assert err <= 1e-14
+    assert np.abs(g[3]).max() == 0.0
+    hs = D.hs
+    mask = np.ones(g.shape, dtype=bool)
+    mask[:, hs:hs + D.nz, hs:hs + D.ny, hs:hs + D.nx] = False
+    assert np.abs(g[mask]).max() == 0.0
