@@ -529,14 +529,38 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   }
 }
 
-// zero the halo of a padded array: the y and z halo rows, and with xh also
-// the x halo cells of the interior rows (which x line kernels write)
-__global__ __launch_bounds__(64) void k_zero_halo(Grid G, bool xh, double* __restrict__ a) {
-  const int ih = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ih >= G.nx + 2 * G.hs) return;
-  const int i = ih - G.hs, j = (int)blockIdx.y - G.hs, k = (int)blockIdx.z - G.hs;
-  if (j >= 0 && j < G.ny && k >= 0 && k < G.nz && (!xh || (i >= 0 && i < G.nx))) return;
-  const int64_t t = (int64_t)blockIdx.z * G.sz + (int64_t)blockIdx.y * G.sy + ih;
+// zero the halo of a padded array, one thread per halo cell: the 2hs z
+// planes, the 2hs y rows of every interior level and, with xh, the 2hs x
+// cells of every interior row (x line kernels write those themselves)
+__host__ __device__ inline int64_t halo_cells(const Grid& G, bool xh) {
+  const int64_t nxh = G.nx + 2 * G.hs, nyh = G.ny + 2 * G.hs, h2 = 2 * G.hs;
+  return h2 * nyh * nxh + (int64_t)G.nz * h2 * nxh + (xh ? (int64_t)G.nz * G.ny * h2 : 0);
+}
+
+__global__ __launch_bounds__(256) void k_zero_halo(Grid G, bool xh, double* __restrict__ a) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= halo_cells(G, xh)) return;
+  const int hs = G.hs, h2 = 2 * hs;
+  const int64_t nxh = G.nx + h2, nyh = G.ny + h2;
+  const int64_t nA = h2 * nyh * nxh, nB = (int64_t)G.nz * h2 * nxh;
+  int64_t kp, jp, ip;                  // padded coordinates
+  if (e < nA) {
+    const int64_t plane = e / (nyh * nxh), r = e % (nyh * nxh);
+    kp = plane < hs ? plane : G.nz + plane;
+    jp = r / nxh;
+    ip = r % nxh;
+  } else if (e < nA + nB) {
+    const int64_t p = e - nA, r = p % (h2 * nxh), row = r / nxh;
+    kp = hs + p / (h2 * nxh);
+    jp = row < hs ? row : G.ny + row;
+    ip = r % nxh;
+  } else {
+    const int64_t p = e - nA - nB, r = p % ((int64_t)G.ny * h2), col = r % h2;
+    kp = hs + p / ((int64_t)G.ny * h2);
+    jp = hs + r / h2;
+    ip = col < hs ? col : G.nx + col;
+  }
+  const int64_t t = kp * G.sz + jp * G.sy + ip;
   sfor<5>([&](auto l) { a[l * G.sl + t] = 0.0; });
 }
 
@@ -752,7 +776,8 @@ int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* 
     const int threads = std::min(256, (n + 63) / 64 * 64);
     v->line[dir]<<<lines, threads, lds, c->stream>>>(in, adj_out, adj_in, G, K);
     LAUNCH_CHECK(c);
-    k_zero_halo<<<padded_grid(c), 64, 0, c->stream>>>(G, dir != 0, adj_in);
+    const int64_t nh = halo_cells(G, dir != 0);
+    k_zero_halo<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(G, dir != 0, adj_in);
     LAUNCH_CHECK(c);
     return AWFL_OK;
   }

@@ -1,6 +1,7 @@
 """Launch each fused sweep kernel a fixed number of times on the thermal
-bubble (default 256x256x128; FP64 sweeps, or the FP32 perturbation-form
-sweeps with --precision fp32), plus one elementwise kernel of known size
+bubble (default 256x256x128; FP64 sweeps, or with --precision the FP32
+perturbation-form sweeps, the adjoint sweeps or the sweeps with the driver's
+three tracers), plus one elementwise kernel of known size
 (read 8N, write 8N bytes) that calibrates HBM byte counters.  Meant to run
 under rocprofv3:
 
@@ -20,8 +21,9 @@ def main():
     ap.add_argument("--ngll", type=int, default=3)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=512)
-    ap.add_argument("--precision", choices=("fp64", "fp32", "adjoint"), default="fp64",
-                    help="fp64 sweeps, FP32 perturbation-form sweeps, or FP64 adjoint sweeps")
+    ap.add_argument("--precision", choices=("fp64", "fp32", "adjoint", "tracers"), default="fp64",
+                    help="fp64 sweeps, FP32 perturbation-form sweeps, FP64 adjoint sweeps, or FP64 sweeps "
+                         "with the driver's three tracers")
     a = ap.parse_args()
     import torch
     from awflcloud_amd import Dycore, bubble_spec
@@ -36,6 +38,13 @@ def main():
         for d in range(3):
             for _ in range(a.reps):
                 D.sweep(st, out, dt, d)
+    elif a.precision == "tracers":
+        D.set_tracers([1, 1, 1])
+        tr = D.init_tracers()
+        out, tro = torch.empty_like(st), torch.empty_like(tr)
+        for d in range(3):
+            for _ in range(a.reps):
+                D.sweep_tracers(st, tr, out, tro, dt, d)
     elif a.precision == "adjoint":
         y = torch.randn_like(st)
         for d in range(3):

@@ -159,6 +159,52 @@ def weno9_leg(steps=50, warmup=2, cfl=0.5):
             "sweep_ms": kms, "finite": bool(torch.isfinite(st).all())}
 
 
+def tracer_leg(steps=20, warmup=2):
+    """The driver's three tracers (driver.cpp:27-29: uniform, theta replica,
+    block; initTracers S:505-560, all positive) transported with the state on
+    the 256x256x128 thermal: fused steps with tracers (awfl_time_step_tracers)
+    against the same steps without, plus the uniform tracer's mixing-ratio
+    error and the tracer mass drift (periodic x/y, walls in z: conserved)."""
+    import torch
+    from awflcloud_amd import Dycore
+    from awflcloud_amd.workloads import thermal_spec
+    nx, ny, nz = 256, 256, 128
+    D = Dycore(nx, ny, nz, bc=(0, 0, 1), ord=5, ngll=3)
+    st = D.init_state(thermal_spec())
+    dt = D.compute_time_step(st, 0.8)
+    D.set_tracers([1, 1, 1])
+    tr = D.init_tracers()
+    ref = st.clone()
+
+    def run(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    D.dim_switch = 1
+    run(lambda: D.time_step(ref, dt), warmup)
+    w0 = run(lambda: D.time_step(ref, dt), steps)
+    m0 = D.tracer_interior(tr).sum(dim=(1, 2, 3))
+    D.dim_switch = 1
+    run(lambda: D.time_step_tracers(st, tr, dt), warmup)
+    w1 = run(lambda: D.time_step_tracers(st, tr, dt), steps)
+    m1 = D.tracer_interior(tr).sum(dim=(1, 2, 3))
+    hs = D.hs
+    rb = torch.tensor(D.background(0)[hs:hs + nz], dtype=torch.float64, device="cuda")
+    q = D.tracer_interior(tr)[0] / (D.interior(st)[0] + rb[:, None, None])
+    ncell = nx * ny * nz
+    return {"workload": "thermal bubble %dx%dx%d, WENO5/ADER3, FP64, 3 positive tracers (uniform, theta replica, "
+                        "block), %d steps" % (nx, ny, nz, steps),
+            "cell_updates_per_s": ncell * steps / w1, "ms_per_step": w1 / steps * 1e3,
+            "ms_per_step_without_tracers": w0 / steps * 1e3, "with_over_without": w1 / w0,
+            "uniform_ratio_max_err": float((q - 1.0).abs().max()),
+            "tracer_mass_drift_rel": float(((m1 - m0).abs() / m0.abs()).max()),
+            "tracer_min": [float(v) for v in D.tracer_interior(tr).amin(dim=(1, 2, 3))]}
+
+
 def adjoint_leg(reps=3):
     """Discrete adjoint on the 128x128x64 bubble: 60 FP64 steps with a
     checkpoint every 10 (awfl_forward_ckpt) against the adjoint run
@@ -215,6 +261,7 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=STEPS4)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing leg")
     ap.add_argument("--no-weno9", action="store_true", help="skip the WENO9/ADER5 leg")
+    ap.add_argument("--no-tracers", action="store_true", help="skip the tracer-transport leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -303,6 +350,9 @@ def main():
     if not args.no_weno9:
         torch.cuda.empty_cache()
         out["weno9"] = weno9_leg()
+    if not args.no_tracers:
+        torch.cuda.empty_cache()
+        out["tracers"] = tracer_leg()
     if not args.no_cpu:
         v1, n1, t1 = cpu_baseline(1, args.cpu_seconds)
         out["cpu_baseline"] = {"value": v1, "unit": "cell-updates/s", "cores": 1, "kind": "port",
