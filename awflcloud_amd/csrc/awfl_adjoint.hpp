@@ -404,30 +404,6 @@ AWFL_HD void tv_adj(const double (&a)[M], double tvb, double (&ab)[M]) {
   });
 }
 
-// The bridge polynomial's coefficients as one linear map of the stencil,
-// [cell][coef]: (recon[hs+1] - sum_i idl_i recon[i] on cells i..i+hs) /
-// idl[hs+1].  The reverse uses it in place of the three statements of
-// weno_coefs (full-stencil polynomial, subtraction, scaling); its values
-// differ from theirs by rounding only.
-template <int ORD>
-struct BridgeMap {
-  double m[ORD][ORD];
-  constexpr BridgeMap() : m{} {
-    using W = Weno<ORD>;
-    constexpr int hs = W::hs;
-    for (int s = 0; s < ORD; s++)
-      for (int ii = 0; ii < ORD; ii++) {
-        double t = W::recon[hs + 1][s][ii];
-        if (ii <= hs)
-          for (int i = 0; i <= hs; i++)
-            if (s - i >= 0 && s - i <= hs) t -= W::idl[i] * W::recon[i][s - i][ii];
-        m[s][ii] = t * (1.0 / W::idl[hs + 1]);
-      }
-  }
-};
-template <int ORD>
-inline constexpr BridgeMap<ORD> kBridge{};
-
 // ub += (d reconstruct / du)^T gb: WENO (weno_coefs, then the projection to
 // GLL points) or the plain projection of the stencil
 template <int ORD, int NGLL>

@@ -85,6 +85,30 @@ __host__ __device__ __forceinline__ void excl_products(const T (&b)[N], T (&ex)[
   P = pre[N - 1] * b[N - 1];
 }
 
+// The bridge polynomial's coefficients as one linear map of the stencil,
+// [cell][coef]: (recon[hs+1] - sum_i idl_i recon[i] on cells i..i+hs) /
+// idl[hs+1], the three statements of WenoLimiter.h:117-131 (full-stencil
+// polynomial, subtraction of the weighted candidates, scaling) folded into
+// one table at compile time; its values differ from theirs by rounding only.
+template <int ORD>
+struct BridgeMap {
+  double m[ORD][ORD];
+  constexpr BridgeMap() : m{} {
+    using W = Weno<ORD>;
+    constexpr int hs = W::hs;
+    for (int s = 0; s < ORD; s++)
+      for (int ii = 0; ii < ORD; ii++) {
+        double t = W::recon[hs + 1][s][ii];
+        if (ii <= hs)
+          for (int i = 0; i <= hs; i++)
+            if (s - i >= 0 && s - i <= hs) t -= W::idl[i] * W::recon[i][s - i][ii];
+        m[s][ii] = t * (1.0 / W::idl[hs + 1]);
+      }
+  }
+};
+template <int ORD>
+inline constexpr BridgeMap<ORD> kBridge{};
+
 // ---------------------------------------------------------------------------
 // weno::compute_weno_coefs: ord cell averages u -> ord polynomial coefficients
 // ---------------------------------------------------------------------------
@@ -104,22 +128,16 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
       a[i][ii] = t;
     });
   });
-  // order-ORD polynomial on the full stencil (:117-121)
+  // "bridge" polynomial (:117-131): the full-stencil polynomial less the
+  // weighted candidates, scaled, as one constant map of the stencil
   sfor<ORD>([&](auto ii) {
     T t = -0.0;
     sfor<ORD>([&](auto s) {
-      constexpr T cf = (T)W::recon[hs + 1][s][ii];
+      constexpr T cf = (T)kBridge<ORD>.m[s][ii];
       if constexpr (cf != 0) t += cf * u[s];
     });
     a[hs + 1][ii] = t;
   });
-  // "bridge" polynomial (:123-131)
-  sfor<hs + 1>([&](auto i) {
-    constexpr T d = (T)W::idl[i];
-    sfor<hs + 1>([&](auto ii) { a[hs + 1][ii] -= d * a[i][ii]; });
-  });
-  constexpr T ridl = (T)(1.0 / W::idl[hs + 1]);
-  sfor<ORD>([&](auto ii) { a[hs + 1][ii] *= ridl; });
   // total variation of every candidate (:133-143), sigma blend (:145-151)
   T tv[hs + 2];
   sfor<hs + 1>([&](auto i) { tv[i] = tv_eval<hs + 1>(a[i]); });
