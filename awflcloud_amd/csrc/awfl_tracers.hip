@@ -73,9 +73,12 @@ __device__ __forceinline__ int wrap_axis(int a, int n, bool per, bool& zn) {
 
 // One line of a DIR sweep with ntr tracers.  LDS (doubles), n cells, W =
 // n + 2hs, NF = n + 1 faces:
-//   state line [5][W], mixing ratios [ntr][W], state edges [10][n], tracer
-//   edges [2 ntr][n] (then the limiter's cell factors), z sources [n],
-//   state fluxes [5][NF], tracer fluxes [ntr][NF], limiter direction [NF]
+//   state line [5][W] (then the state fluxes [5][NF]), mixing ratios
+//   [ntr][W], state edges [10][n], tracer edges [2 ntr][n] (then the
+//   limiter's cell factors), z sources [n], cell densities [n], tracer
+//   fluxes [ntr][NF], limiter direction [NF].
+// Divisions by the density and by the edge densities are multiplications by
+// one reciprocal (rcp, ~1 ulp) shared by all tracers.
 template <int ORD, int NGLL, int DIR, bool TEND>
 __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in, const double* __restrict__ tr_in,
                                                   double* __restrict__ out, double* __restrict__ tr_out, Grid G,
@@ -87,7 +90,6 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
   const int n = axis_n(G, DIR);
   const int W = n + 2 * hs;
   const int NF = n + 1;
-  const double d = DIR == 0 ? G.dx : DIR == 1 ? G.dy : G.dz;
   // line -> first interior cell, stride along DIR, level of an x or y line
   const unsigned line = tile_index();
   int64_t base, stride;
@@ -109,17 +111,16 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
     stride = G.sz;
   }
   double* s_st = sm;                   // [5][W]
+  double* s_F = sm;                    // [5][NF], over the state line once the cells are done
   double* s_tq = s_st + 5 * W;         // [ntr][W]
   double* s_e = s_tq + ntr * W;        // [10][n]
   double* s_te = s_e + 10 * n;         // [2 ntr][n]
   double* s_src = s_te + 2 * ntr * n;  // [n]
-  double* s_F = s_src + n;             // [5][NF]
-  double* s_TF = s_F + 5 * NF;         // [ntr][NF]
+  double* s_dens = s_src + n;          // [n]
+  double* s_TF = s_dens + n;           // [ntr][NF]
   double* s_ul = s_TF + ntr * NF;      // [NF]
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   auto level = [&](int c) { return DIR == 2 ? c : kl; };
-  // full density of stored cell c (S:624-626)
-  auto dens = [&](int c) { return s_st[idR * W + c + hs] + G.hyDensCells[hs + level(c)]; };
 
   // 1. the state line (normal momentum zero in wall ghosts), then the
   // tracers' mixing ratios (ghosts copy the end cell at a wall)
@@ -129,11 +130,14 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
     sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
   }
   __syncthreads();
+  // full density of every cell (S:624-626)
+  for (int c = threadIdx.x; c < n; c += blockDim.x) s_dens[c] = s_st[idR * W + c + hs] + G.hyDensCells[hs + level(c)];
+  __syncthreads();
   for (int c = threadIdx.x; c < W; c += blockDim.x) {
     bool zn;
     const int src = wrap_axis(c - hs, n, per, zn);
-    const double rho = dens(src);
-    for (int t = 0; t < ntr; t++) s_tq[t * W + c] = tr_in[t * G.sl + base + src * stride] / rho;
+    const double rr = rcp(s_dens[src]);
+    for (int t = 0; t < ntr; t++) s_tq[t * W + c] = tr_in[t * G.sl + base + src * stride] * rr;
   }
   __syncthreads();
 
@@ -172,6 +176,8 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
       s_e[(5 + l) * n + c] = eR[l];
     });
     s_src[c] = sa;
+    double rr0[NGLL];
+    sfor<NGLL>([&](auto ii) { rr0[ii] = rcp(rl[0][ii]); });
     for (int t = 0; t < ntr; t++) {
       const bool tp = (pos >> t) & 1u;
       double u[ORD], g[NGLL];
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
       sfor<NGLL>([&](auto ii) {
         rt[0][ii] = g[ii] * rl[0][ii];
         if (tp) rt[0][ii] = fmax(0.0, rt[0][ii]);
-        rnt[0][ii] = rt[0][ii] * nl[0][ii] / rl[0][ii];
+        rnt[0][ii] = rt[0][ii] * nl[0][ii] * rr0[ii];
       });
       // diffTransformTracer (S:2095-2126)
       sfor<NA - 1>([&](auto kt) {
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
               if constexpr (ir == 0) tot += nl[ir][ii] * rt[kt + 1 - ir][ii];
               else tot += nl[ir][ii] * rt[kt + 1 - ir][ii] - rl[ir][ii] * rnt[kt + 1 - ir][ii];
             });
-            rnt[kt + 1][ii] = tot / rl[0][ii];
+            rnt[kt + 1][ii] = tot * rr0[ii];
           });
         }
       });
@@ -248,12 +254,13 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
       else F[idW] -= G.hyPressureGLL[(G.nz - 1) * NGLL + NGLL - 1];
     }
     sfor<5>([&](auto l) { s_F[l * NF + f] = F[l]; });
+    const double up = un > 0 ? F[0] * rcp(L[0]) : F[0] * rcp(R[0]);
     for (int t = 0; t < ntr; t++) {
       // rho*q of the upwind side: the low cell's high edge, or the high
       // cell's low edge (a wall face takes both from its one cell)
       const double tL = wl ? s_te[(2 * t) * n + 0] : s_te[(2 * t + 1) * n + a];
       const double tR = wr ? s_te[(2 * t + 1) * n + n - 1] : s_te[(2 * t) * n + b];
-      s_TF[t * NF + f] = un > 0 ? F[0] * tL / L[0] : F[0] * tR / R[0];
+      s_TF[t * NF + f] = (un > 0 ? tL : tR) * up;
     }
     s_ul[f] = 0.5 * (L[NM] + R[NM]);
   }
@@ -269,8 +276,8 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
         if (!((pos >> t) & 1u)) continue;
         const double f1 = fmin(s_TF[t * NF + c], 0.0);
         const double f2 = fmax(s_TF[t * NF + cp], 0.0);
-        const double fluxOut = K.dt * (f2 - f1) / d;
-        s_fac[t * n + c] = fmin(1.0, s_tq[t * W + c + hs] * dens(c) / (fluxOut + TRACER_EPS));
+        const double fluxOut = K.dt * (f2 - f1) * K.rd;
+        s_fac[t * n + c] = fmin(1.0, s_tq[t * W + c + hs] * s_dens[c] * rcp(fluxOut + TRACER_EPS));
       }
     }
     __syncthreads();
@@ -287,7 +294,7 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
   }
 
   // 5. tendencies: the state as the production sweep forms them, the
-  // tracers as -(flux difference) / d (S:950)
+  // tracers as -(flux difference) / d (S:950) by the same reciprocal
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
     const int cp = (wrapF && c + 1 == n) ? 0 : c + 1;
     int k = kl, j = jl, i = il;
@@ -305,7 +312,7 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
       else out[l * G.sl + o] = in[l * G.sl + o] + K.dt * td;
     });
     for (int t = 0; t < ntr; t++) {
-      const double td = -(s_TF[t * NF + cp] - s_TF[t * NF + c]) / d;
+      const double td = -(s_TF[t * NF + cp] - s_TF[t * NF + c]) * K.rd;
       if constexpr (TEND) tr_out[t * nc + ot] = td;
       else tr_out[t * G.sl + o] = tr_in[t * G.sl + o] + K.dt * td;
     }
@@ -400,7 +407,7 @@ int launch_sweep_tr(awfl_ctx* c, const double* in, const double* tr_in, double* 
   if (!v) return fail(c, AWFL_E_UNSUPPORTED, "tracers: (ord, ngll) not compiled in");
   const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
   const int W = n + 2 * c->hs, ntr = c->ntr;
-  const size_t lds = sizeof(double) * (size_t)(5 * W + ntr * W + 10 * n + 2 * ntr * n + n + (5 + ntr + 1) * (n + 1));
+  const size_t lds = sizeof(double) * (size_t)(5 * W + ntr * W + 10 * n + 2 * ntr * n + 2 * n + (ntr + 1) * (n + 1));
   if (lds > 160 * 1024) return fail(c, AWFL_E_UNSUPPORTED, "tracers: line too long for the tracer sweep's LDS");
   KernelTr k = tend ? v->tend[dir] : v->upd[dir];
   if (lds > 64 * 1024)
