@@ -76,8 +76,11 @@ CASES = [
     dict(n=(24, 20, 12), bc=(0, 0, 0)),
     dict(n=(20, 16, 12), bc=(0, 1, 1), ord=3, ngll=2),
     dict(n=(32, 1, 16), bc=(0, 0, 1)),
+    dict(n=(24, 20, 16), bc=(0, 0, 1), ord=9, ngll=5),
 ]
 IDS = ["x".join(map(str, c["n"])) + "-bc" + "".join(map(str, c["bc"])) + "-o%d" % c.get("ord", 5) for c in CASES]
+# ten steps at CFL 0.8: the WENO5 cases (WENO9/ADER5 needs a smaller CFL over many steps)
+STEP_CASES = [0, 1, 2, 4]
 
 
 @pytest.mark.parametrize("case", CASES, ids=IDS)
@@ -118,7 +121,7 @@ def test_tendencies_match_oracle(case, direction):
     assert np.abs(I(got) - I(upd)).max() <= 4e-16 * np.abs(I(upd)).max()
 
 
-@pytest.mark.parametrize("case", CASES[:3] + CASES[4:], ids=IDS[:3] + IDS[4:])
+@pytest.mark.parametrize("case", [CASES[i] for i in STEP_CASES], ids=[IDS[i] for i in STEP_CASES])
 def test_time_steps_match_oracle(case):
     _need_gpu()
     print("case", case, flush=True)
