@@ -57,10 +57,12 @@ def cpu_baseline(nthreads, target_s=12.0):
     return nx * ny * nz * steps / t, steps, t
 
 
-def fp32_leg(steps, warmup):
+def fp32_leg(steps, warmup, chunk=20):
     """FP64 and FP32 runs of the 512x512x256 workload, each through its fused
     C-ABI time step (awfl_time_step / awfl_time_step_f32); throughput of each
-    and the FP32 errors of theta', w and rho' against the FP64 run."""
+    and the FP32 errors of theta', w and rho' against the FP64 run.  The two
+    runs advance in alternating chunks of `chunk` steps, each chunk timed on
+    its own, so that both see the same clocks and temperature."""
     import torch
     from awflcloud_amd import Dycore, bubble_spec, band_limited_modes
     from awflcloud_amd.dycore import AWFL_FP32_PERT
@@ -71,33 +73,43 @@ def fp32_leg(steps, warmup):
     st0 = D.init_state(spec)
     dt = D.compute_time_step(st0, 0.8)
 
+    # the two runs keep their own dimSwitch sequence (both start at 1)
+    ds = {"64": 1, "32": 1}
+
     def run64(st, n):
-        D.dim_switch = 1
+        D.dim_switch = ds["64"]
         for _ in range(n):
             D.time_step(st, dt)
+        ds["64"] = D.dim_switch
 
     def run32(s, c, n):
-        D.dim_switch = 1
+        D.dim_switch = ds["32"]
         for _ in range(n):
             D.time_step_f32(s, c, dt)
+        ds["32"] = D.dim_switch
+
+    def timed(fn, *a):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(*a)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
 
     scratch = st0.clone()
     run64(scratch, warmup)
     s, c = D.to_f32(st0)
     run32(s, c, warmup)
     del scratch
+    ds["64"] = ds["32"] = 1
     st64 = st0.clone()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run64(st64, steps)
-    torch.cuda.synchronize()
-    w64 = time.perf_counter() - t0
     s, c = D.to_f32(st0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run32(s, c, steps)
-    torch.cuda.synchronize()
-    w32 = time.perf_counter() - t0
+    w64 = w32 = 0.0
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        w64 += timed(run64, st64, n)
+        w32 += timed(run32, s, c, n)
+        done += n
     st32 = D.from_f32(s, c)
     del s, c
 
