@@ -355,6 +355,7 @@ template <int ORD, int NGLL>
 constexpr int adj_line_rows() {
   return 5 * NGLL + 10;
 }
+constexpr int LINE_K = 10;
 
 template <int ORD, int NGLL, int DIR>
 __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in, const double* __restrict__ adj_out,
@@ -389,10 +390,44 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   double* s_st = sm;                 // [5][W]
   double* s_q = s_st + 5 * W;        // [5 NGLL][n]: q0, then q0b
   double* s_e = s_q + 5 * NGLL * n;  // [10][n]: eL|eR, then eLb|eRb, then stencil adjoints
+  double* s_k = s_e + 10 * n;        // [LINE_K]: background and power references of an x or y line
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   const double* ao = adj_out + base;
-  // background of cell c: the line's level for x and y, the cell's own for z
-  auto bg_of = [&](int c) { return cell_bg<NGLL, DIR>(G, DIR == 2 ? c : kl); };
+  // background of cell c: the line's level for x and y (read from LDS, so
+  // that these per-line values live in vector registers only where they are
+  // used), the cell's own for z
+  auto bg_of = [&](int c) {
+    if constexpr (DIR == 2) {
+      return cell_bg<NGLL, DIR>(G, c);
+    } else {
+      CellBg<NGLL> bg{};
+      bg.r = s_k[0];
+      bg.rt = s_k[1];
+      bg.pt = PowRef{s_k[2], s_k[3], s_k[4], s_k[5]};
+      bg.wall_bot = bg.wall_top = false;
+      return bg;
+    }
+  };
+  auto fref_of = [&](int f) {
+    if constexpr (DIR == 2) {
+      return face_ref<DIR>(G, kl, f);
+    } else {
+      FaceRef fr;
+      fr.q5 = PowRef{s_k[2], s_k[3], s_k[4], s_k[5]};
+      fr.rt = DIR == 0 ? fr.q5 : PowRef{s_k[6], s_k[7], s_k[8], s_k[9]};
+      return fr;
+    }
+  };
+  if constexpr (DIR != 2) {
+    if (threadIdx.x == 0) {
+      const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, kl);
+      const FaceRef fr = face_ref<DIR>(G, kl, 0);
+      s_k[0] = bg.r;
+      s_k[1] = bg.rt;
+      s_k[2] = bg.pt.h, s_k[3] = bg.pt.l, s_k[4] = bg.pt.rb, s_k[5] = bg.pt.b;
+      s_k[6] = fr.rt.h, s_k[7] = fr.rt.l, s_k[8] = fr.rt.rb, s_k[9] = fr.rt.b;
+    }
+  }
   // 1. stage the line (boundary rule at load)
   for (int c = threadIdx.x; c < W; c += blockDim.x) {
     bool zn;
@@ -400,15 +435,23 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
   }
   __syncthreads();
-  // 2. cell forward: WENO values (one variable at a time) and edge states
+  // 2. cell forward: WENO values (two variables at a time, independent
+  // chains the scheduler interleaves) and edge states
+  auto dow = [&](int l) { return (l == idR || l == idT) ? G.weno_s : G.weno_w; };
+  auto recon = [&](int c, int l0, auto nv) {
+    constexpr int NV = decltype(nv)::value;
+    double w[NV][ORD], g[NV][NGLL];
+    sfor<NV>([&](auto v) { sfor<ORD>([&](auto s) { w[v][s] = s_st[(l0 + v) * W + c + s]; }); });
+    bool all = true;
+    sfor<NV>([&](auto v) { all = all && dow(l0 + v); });
+    if (all) sfor<NV>([&](auto v) { reconstruct<ORD, NGLL>(w[v], true, g[v]); });
+    else sfor<NV>([&](auto v) { reconstruct<ORD, NGLL>(w[v], dow(l0 + v), g[v]); });
+    sfor<NV>([&](auto v) { sfor<NGLL>([&](auto ii) { s_q[((l0 + v) * NGLL + ii) * n + c] = g[v][ii]; }); });
+  };
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
 #pragma unroll 1
-    for (int l = 0; l < 5; l++) {
-      double w[ORD], g[NGLL];
-      sfor<ORD>([&](auto s) { w[s] = s_st[l * W + c + s]; });
-      reconstruct<ORD, NGLL>(w, (l == idR || l == idT) ? G.weno_s : G.weno_w, g);
-      sfor<NGLL>([&](auto ii) { s_q[(l * NGLL + ii) * n + c] = g[ii]; });
-    }
+    for (int l0 = 0; l0 < 4; l0 += 2) recon(c, l0, std::integral_constant<int, 2>{});
+    recon(c, 4, std::integral_constant<int, 1>{});
     double q0[5][NGLL], eL[5], eR[5], sa;
     sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { q0[l][ii] = s_q[(l * NGLL + ii) * n + c]; }); });
     adj::Tape<NGLL, double> tp;
@@ -440,7 +483,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       Fb[l] = (G.sim2d && l == idV) ? 0.0 : cdr * v;
       Lb[l] = Rb[l] = 0.0;
     });
-    adj::riemann_adj<DIR>(L, R, face_ref<DIR>(G, kl, f), Fb, Lb, Rb);
+    adj::riemann_adj<DIR>(L, R, fref_of(f), Fb, Lb, Rb);
     if (wl || wr) {
       sfor<5>([&](auto l) { s_e[sL + l * n] = Lb[l] + Rb[l]; });
     } else {
@@ -464,62 +507,97 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     adj::cell_dt_adj<NGLL, DIR>(q0, bg_of(c), K, eLb, eRb, srcb, q0b);
     sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { s_q[(l * NGLL + ii) * n + c] = q0b[l][ii]; }); });
   }
-  // 4b. one variable at a time: WENO reverse into the edge region (each
-  // cell's own column, whose edge adjoints step 4a has consumed), then the
-  // gather of the stencil adjoints that point at each cell
-#pragma unroll 1
-  for (int l = 0; l < 5; l++) {
-    const bool doweno = (l == idR || l == idT) ? G.weno_s : G.weno_w;
+  // 4b. WENO reverse into the edge region (each cell's own column, whose
+  // edge adjoints step 4a has consumed), PV variables at a time side by side
+  // (two independent reverses per thread give the scheduler two chains to
+  // interleave), then the gather of the stencil adjoints that point at each
+  // cell
+  constexpr int PV = 2 * ORD <= 10 ? 2 : 1;
+  // variables l0 .. l0+NV-1: stencil adjoints of variable l0+v in rows v*ORD+s
+  auto weno_rev = [&](int l0, auto nv) {
+    constexpr int NV = decltype(nv)::value;
+    bool all = true;
+    sfor<NV>([&](auto v) { all = all && dow(l0 + v); });
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
-      double u[ORD], ub[ORD], gb[NGLL];
-      sfor<ORD>([&](auto s) {
-        u[s] = s_st[l * W + c + s];
-        ub[s] = 0.0;
+      double u[NV][ORD], ub[NV][ORD], gb[NV][NGLL];
+      sfor<NV>([&](auto v) {
+        sfor<ORD>([&](auto s) {
+          u[v][s] = s_st[(l0 + v) * W + c + s];
+          ub[v][s] = 0.0;
+        });
+        sfor<NGLL>([&](auto ii) { gb[v][ii] = s_q[((l0 + v) * NGLL + ii) * n + c]; });
       });
-      sfor<NGLL>([&](auto ii) { gb[ii] = s_q[(l * NGLL + ii) * n + c]; });
-      adj::reconstruct_adj<ORD, NGLL>(u, doweno, gb, ub);
-      sfor<ORD>([&](auto s) { s_e[s * n + c] = ub[s]; });
-    }
-    __syncthreads();
-    // at a wall the stencil slots past the end read the clamped end cell;
-    // the normal momentum there is zero, so it passes nothing back
-    const bool ghost = !per && l != NM;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-      double acc = ao[l * G.sl + c * stride];
-      sfor<ORD>([&](auto s) {
-        int src = c + hs - s;            // the cell whose stencil slot s is c
-        bool ok = true;
-        if (per) {
-          while (src < 0) src += n;
-          while (src >= n) src -= n;
-        } else {
-          ok = src >= 0 && src < n;
-        }
-        if (ok) acc += s_e[s * n + src];
-      });
-      if (ghost) {
-        if (c == 0)                      // slot s of cell cc reads index cc - hs + s < 0
-          sfor<hs>([&](auto cc) {
-            sfor<ORD>([&](auto s) {
-              if constexpr (cc - hs + s < 0) {
-                if (cc < n) acc += s_e[s * n + cc];
-              }
-            });
-          });
-        if (c == n - 1)                  // slot s of cell n-1-m reads index >= n
-          sfor<hs>([&](auto m) {
-            const int cc = n - 1 - m;
-            sfor<ORD>([&](auto s) {
-              if constexpr (s >= hs + 1 + m) {
-                if (cc >= 0) acc += s_e[s * n + cc];
-              }
-            });
-          });
+      if (all) {
+        sfor<NV>([&](auto v) { adj::reconstruct_adj<ORD, NGLL>(u[v], true, gb[v], ub[v]); });
+      } else {
+        sfor<NV>([&](auto v) { adj::reconstruct_adj<ORD, NGLL>(u[v], dow(l0 + v), gb[v], ub[v]); });
       }
-      adj_in[l * G.sl + base + c * stride] = acc;
+      sfor<NV>([&](auto v) { sfor<ORD>([&](auto s) { s_e[(v * ORD + s) * n + c] = ub[v][s]; }); });
     }
+  };
+  // src = c + hs - s is the cell whose stencil slot s is c: a periodic line
+  // wraps it (by one period when n >= hs, else by the remainder), a wall
+  // drops it when it falls outside the line.  At a wall the stencil slots
+  // past the end read the clamped end cell; the normal momentum there is
+  // zero, so it passes nothing back.
+  const int wrap = per ? n : 0;
+  auto gather = [&](int l0, auto nv, auto small) {
+    constexpr int NV = decltype(nv)::value;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      sfor<NV>([&](auto v) {
+        const int l = l0 + v;
+        const double* se = s_e + v * ORD * n;
+        double acc = ao[l * G.sl + c * stride];
+        sfor<ORD>([&](auto s) {
+          if constexpr (s == hs) {
+            acc += se[s * n + c];
+          } else {
+            int src = c + hs - s;
+            if constexpr (decltype(small)::value) {
+              src %= n;
+              src += src < 0 ? n : 0;
+            } else {
+              src += src < 0 ? wrap : 0;
+              src -= src >= n ? wrap : 0;
+            }
+            const bool ok = src >= 0 && src < n;
+            const double val = se[s * n + (ok ? src : c)];
+            acc += ok ? val : 0.0;
+          }
+        });
+        if (!per && l != NM) {
+          if (c == 0)                    // slot s of cell cc reads index cc - hs + s < 0
+            sfor<hs>([&](auto cc) {
+              sfor<ORD>([&](auto s) {
+                if constexpr (cc - hs + s < 0) {
+                  if (cc < n) acc += se[s * n + cc];
+                }
+              });
+            });
+          if (c == n - 1)                // slot s of cell n-1-m reads index >= n
+            sfor<hs>([&](auto m) {
+              const int cc = n - 1 - m;
+              sfor<ORD>([&](auto s) {
+                if constexpr (s >= hs + 1 + m) {
+                  if (cc >= 0) acc += se[s * n + cc];
+                }
+              });
+            });
+        }
+        adj_in[l * G.sl + base + c * stride] = acc;
+      });
+    }
+  };
+  auto step = [&](int l0, auto nv) {
+    weno_rev(l0, nv);
     __syncthreads();
-  }
+    if (per && n < hs) gather(l0, nv, std::true_type{});
+    else gather(l0, nv, std::false_type{});
+    __syncthreads();
+  };
+#pragma unroll 1
+  for (int l0 = 0; l0 + PV <= 5; l0 += PV) step(l0, std::integral_constant<int, PV>{});
+  if constexpr (5 % PV != 0) step(5 - 5 % PV, std::integral_constant<int, 5 % PV>{});
   // the halo cells of an x line (y and z halos: k_zero_halo)
   if constexpr (DIR == 0) {
     for (int c = threadIdx.x; c < 2 * hs; c += blockDim.x) {
@@ -742,7 +820,7 @@ int ensure_tmp(awfl_ctx* c) {
 // LDS bytes of the fused line kernel along dir (0 if it does not fit)
 size_t line_lds(const awfl_ctx* c, const VariantA* v, int dir) {
   const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
-  const size_t b = sizeof(double) * (size_t)(5 * (n + 2 * c->hs) + v->line_rows * n);
+  const size_t b = sizeof(double) * (size_t)(5 * (n + 2 * c->hs) + v->line_rows * n + LINE_K);
   return b <= 160 * 1024 ? b : 0;
 }
 
