@@ -833,7 +833,11 @@ dim3 padded_grid(const awfl_ctx* c) {
 }
 
 // adj_in = adj_out + J(in)^T (dt adj_out) for one sweep
-int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* adj_in, double dt, int dir) {
+// zero_halo: leave the halo of adj_in zero (one extra kernel; x lines write
+// their own x halo cells).  Chains of adjoint sweeps read only interior
+// cells, so they zero the halo once at the end instead.
+int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* adj_in, double dt, int dir,
+                  bool zero_halo = true) {
   if (dir == 1 && c->sim2d) {
     HIP_TRY(c, hipMemcpyAsync(adj_in, adj_out, sizeof(double) * c->state_elems(), hipMemcpyDeviceToDevice,
                               c->stream));
@@ -854,9 +858,11 @@ int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* 
     const int threads = std::min(256, (n + 63) / 64 * 64);
     v->line[dir]<<<lines, threads, lds, c->stream>>>(in, adj_out, adj_in, G, K);
     LAUNCH_CHECK(c);
-    const int64_t nh = halo_cells(G, dir != 0);
-    k_zero_halo<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(G, dir != 0, adj_in);
-    LAUNCH_CHECK(c);
+    if (zero_halo) {
+      const int64_t nh = halo_cells(G, dir != 0);
+      k_zero_halo<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(G, dir != 0, adj_in);
+      LAUNCH_CHECK(c);
+    }
     return AWFL_OK;
   }
   rc = ensure_cell_scratch(c);
@@ -912,12 +918,18 @@ int reverse_steps(awfl_ctx* c, int n0, int n1, int ds0, double dt, double* adj) 
     const int nd = split_dirs(c, ds, dirs);
     for (int k = nd - 1; k >= 0; k--) {
       const double* in = c->a_states + (int64_t)((n - n0) * nd + k) * el;
-      int rc = sweep_adjoint(c, in, cur, oth, dt, dirs[k]);
+      int rc = sweep_adjoint(c, in, cur, oth, dt, dirs[k], false);
       if (rc != AWFL_OK) return rc;
       std::swap(cur, oth);
     }
   }
   if (cur != adj) HIP_TRY(c, hipMemcpyAsync(adj, cur, sizeof(double) * el, hipMemcpyDeviceToDevice, c->stream));
+  if (n1 > n0) {
+    const Grid G = c->grid();
+    const int64_t nh = halo_cells(G, true);
+    k_zero_halo<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(G, true, adj);
+    LAUNCH_CHECK(c);
+  }
   return AWFL_OK;
 }
 

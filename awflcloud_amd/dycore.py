@@ -398,11 +398,16 @@ class Dycore:
                                               self._ptr(dout), dt, direction))
         return out, dout
 
-    def sweep_adjoint(self, state, adj_out, dt, direction):
-        """J(state)^T adj_out for one split sweep"""
+    def sweep_adjoint(self, state, adj_out, dt, direction, out=None):
+        """J(state)^T adj_out for one split sweep, into `out` (a new state
+        when None; its halo comes back zero)"""
         self._chk_state(state)
         self._chk_state(adj_out)
-        adj_in = self.new_state()
+        if out is None:
+            adj_in = self.new_state()
+        else:
+            self._chk_state(out)
+            adj_in = out
         self._check(self.L.awfl_sweep_adjoint(self.h, self._ptr(state), self._ptr(adj_out), self._ptr(adj_in),
                                               dt, direction))
         return adj_in

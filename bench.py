@@ -249,11 +249,12 @@ def adjoint_leg(reps=3):
     sweep = {}
     out = torch.empty_like(base)
     y = torch.randn_like(base)
+    yin = torch.empty_like(base)
     for d in range(3):
         D.sweep(base, out, dt, d)
-        D.sweep_adjoint(base, y, dt, d)
+        D.sweep_adjoint(base, y, dt, d, out=yin)
         tf = timed(lambda: [D.sweep(base, out, dt, d) for _ in range(10)]) / 10
-        ta = timed(lambda: [D.sweep_adjoint(base, y, dt, d) for _ in range(10)]) / 10
+        ta = timed(lambda: [D.sweep_adjoint(base, y, dt, d, out=yin) for _ in range(10)]) / 10
         sweep["xyz"[d]] = {"forward_ms": tf * 1e3, "adjoint_ms": ta * 1e3, "adjoint_over_forward": ta / tf}
     return {"workload": "thermal bubble 128x128x64, WENO5/ADER3, FP64, %d steps, checkpoint every %d"
                         % (nsteps, every),

@@ -149,6 +149,36 @@ def test_time_step_adjoint_is_reverse_of_sweeps():
         assert err <= 1e-14
 
 
+def test_adjoint_output_buffer_and_halo():
+    """An adjoint sweep into a caller's buffer full of NaN equals the one into
+    a fresh buffer, halo included (zero); the input adjoint's halo is not
+    read; the time-step adjoint, which zeroes the halo once after its chain
+    of sweeps, leaves it zero too."""
+    _need_gpu()
+    D, st, dt = _setup(24, 20, 16)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(6)
+    y = _rand_like_state(D, gen)
+    hs = D.hs
+    halo = torch.ones(D.state_shape, dtype=torch.bool, device="cuda")
+    halo[:, hs:hs + D.nz, hs:hs + D.ny, hs:hs + D.nx] = False
+    y7 = y.clone()
+    y7[halo] = 7.0
+    for d in range(3):
+        ref = D.sweep_adjoint(st, y, dt, d)
+        buf = torch.full_like(st, float("nan"))
+        got = D.sweep_adjoint(st, y7, dt, d, out=buf)
+        assert got is buf
+        assert torch.equal(got, ref)
+        assert float(ref[halo].abs().max()) == 0.0
+    for ds in (1, 0):
+        a, b = y.clone(), y7.clone()
+        D.time_step_adjoint(st, a, dt, ds)
+        D.time_step_adjoint(st, b, dt, ds)
+        assert torch.equal(a, b)
+        assert float(a[halo].abs().max()) == 0.0
+
+
 def test_checkpointed_adjoint_equals_stepwise():
     """awfl_adjoint_ckpt (segments of 4) == the step-by-step adjoint"""
     _need_gpu()
