@@ -12,7 +12,10 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libawfl_dycore.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 arithmetic stays off in every HIP build of this package.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+# -fno-signed-zeros lets the compiler drop the addition of a zero-initialised
+# accumulator's first term (x + 0.0 is not x only for x = -0.0); no result
+# changes except the sign of an exact zero, which nothing here depends on.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fno-signed-zeros",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 # translation units of libawfl_dycore.so: FP64 sweeps + ABI, FP32 perturbation form, adjoint,
 # alternative time integrators, tracers

@@ -7,7 +7,7 @@
 // adj_in = adj_out + J^T (dt adj_out) runs as one fused kernel per line
 // along the sweep direction (k_adj_line: forward, face adjoints, transform
 // and WENO reverses and the gather, all in LDS).  Lines too long for LDS
-// (more than about 680 cells at WENO5) run as five kernels over per-cell
+// (more than about 580 cells at WENO5) run as five kernels over per-cell
 // scratch in HBM:
 //   k_edges      cell forward, one thread per cell: WENO values at the GLL
 //                points [5 ngll/cell], then time-averaged edge states and
@@ -348,8 +348,8 @@ __global__ __launch_bounds__(64) void k_weno_adj(const double* __restrict__ st, 
 // cell and then, in place, their adjoints q0b, and a region [10][n] for the
 // edge states, replaced in place by the edge-state adjoints (every slot is
 // read and written by the one face thread that owns it) and then reused,
-// one variable at a time, for the stencil adjoints.  adj_out is read from
-// global memory where it is needed.
+// two variables at a time, for the stencil adjoints; then the line's
+// constants and adj_out along the line [5][n], staged once.
 // ---------------------------------------------------------------------------
 template <int ORD, int NGLL>
 constexpr int adj_line_rows() {
@@ -391,8 +391,8 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   double* s_q = s_st + 5 * W;        // [5 NGLL][n]: q0, then q0b
   double* s_e = s_q + 5 * NGLL * n;  // [10][n]: eL|eR, then eLb|eRb, then stencil adjoints
   double* s_k = s_e + 10 * n;        // [LINE_K]: background and power references of an x or y line
+  double* s_ao = s_k + LINE_K;       // [5][n]: adj_out along the line
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
-  const double* ao = adj_out + base;
   // background of cell c: the line's level for x and y (read from LDS, so
   // that these per-line values live in vector registers only where they are
   // used), the cell's own for z
@@ -428,15 +428,18 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       s_k[6] = fr.rt.h, s_k[7] = fr.rt.l, s_k[8] = fr.rt.rb, s_k[9] = fr.rt.b;
     }
   }
-  // 1. stage the line (boundary rule at load)
+  // 1. stage the line (boundary rule at load) and adj_out along it
   for (int c = threadIdx.x; c < W; c += blockDim.x) {
     bool zn;
     const int src = map_axis(c - hs, n, per, zn);
     sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
   }
+  for (int c = threadIdx.x; c < n; c += blockDim.x)
+    sfor<5>([&](auto l) { s_ao[l * n + c] = adj_out[l * G.sl + base + c * stride]; });
   __syncthreads();
-  // 2. cell forward: WENO values (two variables at a time, independent
-  // chains the scheduler interleaves) and edge states
+  // 2. cell forward: WENO values (independent chains the scheduler
+  // interleaves: two variables at a time in x, all five in y and z, as
+  // measured fastest) and edge states
   auto dow = [&](int l) { return (l == idR || l == idT) ? G.weno_s : G.weno_w; };
   auto recon = [&](int c, int l0, auto nv) {
     constexpr int NV = decltype(nv)::value;
@@ -449,9 +452,13 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     sfor<NV>([&](auto v) { sfor<NGLL>([&](auto ii) { s_q[((l0 + v) * NGLL + ii) * n + c] = g[v][ii]; }); });
   };
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    if constexpr (DIR == 0) {
 #pragma unroll 1
-    for (int l0 = 0; l0 < 4; l0 += 2) recon(c, l0, std::integral_constant<int, 2>{});
-    recon(c, 4, std::integral_constant<int, 1>{});
+      for (int l0 = 0; l0 < 4; l0 += 2) recon(c, l0, std::integral_constant<int, 2>{});
+      recon(c, 4, std::integral_constant<int, 1>{});
+    } else {
+      recon(c, 0, std::integral_constant<int, 5>{});
+    }
     double q0[5][NGLL], eL[5], eR[5], sa;
     sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { q0[l][ii] = s_q[(l * NGLL + ii) * n + c]; }); });
     adj::Tape<NGLL, double> tp;
@@ -478,8 +485,8 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       L[l] = s_e[sL + l * n];
       R[l] = s_e[sR + l * n];
       double v = 0.0;
-      if (!wr) v += ao[l * G.sl + b * stride];
-      if (!wl) v -= ao[l * G.sl + a * stride];
+      if (!wr) v += s_ao[l * n + b];
+      if (!wl) v -= s_ao[l * n + a];
       Fb[l] = (G.sim2d && l == idV) ? 0.0 : cdr * v;
       Lb[l] = Rb[l] = 0.0;
     });
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       eLb[l] = s_e[l * n + c];
       eRb[l] = s_e[(5 + l) * n + c];
     });
-    const double srcb = DIR == 2 ? K.dt * ao[idW * G.sl + c * stride] : 0.0;
+    const double srcb = DIR == 2 ? K.dt * s_ao[idW * n + c] : 0.0;
     adj::cell_dt_adj<NGLL, DIR>(q0, bg_of(c), K, eLb, eRb, srcb, q0b);
     sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { s_q[(l * NGLL + ii) * n + c] = q0b[l][ii]; }); });
   }
@@ -547,7 +554,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       sfor<NV>([&](auto v) {
         const int l = l0 + v;
         const double* se = s_e + v * ORD * n;
-        double acc = ao[l * G.sl + c * stride];
+        double acc = s_ao[l * n + c];
         sfor<ORD>([&](auto s) {
           if constexpr (s == hs) {
             acc += se[s * n + c];
@@ -820,7 +827,8 @@ int ensure_tmp(awfl_ctx* c) {
 // LDS bytes of the fused line kernel along dir (0 if it does not fit)
 size_t line_lds(const awfl_ctx* c, const VariantA* v, int dir) {
   const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
-  const size_t b = sizeof(double) * (size_t)(5 * (n + 2 * c->hs) + v->line_rows * n + LINE_K);
+  // state line, row regions, line constants, adj_out along the line
+  const size_t b = sizeof(double) * (size_t)(5 * (n + 2 * c->hs) + v->line_rows * n + LINE_K + 5 * n);
   return b <= 160 * 1024 ? b : 0;
 }
 
