@@ -171,8 +171,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     int src;
     bool zn = false;
     if (per) {
-      src = a % n;
-      if (src < 0) src += n;
+      if (n > hs) {                    // the walk reaches at most hs + 1 cells past either end
+        src = a < 0 ? a + n : (a >= n ? a - n : a);
+      } else {
+        src = a % n;
+        if (src < 0) src += n;
+      }
     } else if (a < 0) {
       src = 0; zn = true;
     } else if (a >= n) {
@@ -217,17 +221,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
       else F[idW] -= G.hyPressureGLL[(G.nz - 1) * NGLL + NGLL - 1];
     }
   };
-  auto emit = [&](int c, const double (&F0)[5], const double (&F1)[5], double src, const double (&cell)[5]) {
+  // result of one cell (new state, or its tendency) into registers, and its store
+  auto update = [&](const double (&F0)[5], const double (&F1)[5], double src, const double (&cell)[5],
+                    double (&res)[5]) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       double td = -(F1[l] - F0[l]) * K.rd;
       if (DIR == 2 && l == idW) td = src + td;
       if (DIR == 2 && G.sim2d && l == idV) td = 0.0;
+      res[l] = TEND ? td : cell[l] + K.dt * td;
+    }
+  };
+  auto put = [&](int c, const double (&res)[5]) {
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
       if constexpr (TEND) {
-        if constexpr (DIR == 1) out[tend_index(G, l, o, c, i)] = td;
-        else out[tend_index(G, l, c, o, i)] = td;
+        if constexpr (DIR == 1) out[tend_index(G, l, o, c, i)] = res[l];
+        else out[tend_index(G, l, c, o, i)] = res[l];
       } else {
-        out[l * G.sl + base + (int64_t)c * sa] = cell[l] + K.dt * td;
+        out[l * G.sl + base + (int64_t)c * sa] = res[l];
       }
     }
   };
@@ -242,10 +254,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
   }
   // software pipeline: the stencil cell of the next iteration is loaded one
-  // iteration ahead
-  double nxt[5];
-  sfor<5>([&](auto l) { nxt[l] = ld(l, s0 + hs); });
+  // iteration ahead, and the result of an iteration is stored at the top of
+  // the next one, ahead of that iteration's loads: a wait for the loads then
+  // never waits for stores issued just before it
+  double nxt[5], res[5];
+  sfor<5>([&](auto l) {
+    nxt[l] = ld(l, s0 + hs);
+    prevF[l] = prevCell[l] = res[l] = 0.0;
+  });
   for (int v = s0; v < s1; v++) {
+    if (v > s0 + 1) put(v - 2, res);
     sfor<5>([&](auto l) {
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
       w[l][ORD - 1] = nxt[l];
@@ -260,7 +278,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     if (!per && v == 0) riemann<DIR>(eL, eL, fr, F);   // wall: both sides from the first cell
     else riemann<DIR>(prevR, eL, fr, F);
     face_bg(v, F);
-    if (v > s0) emit(v - 1, prevF, F, prevSrc, prevCell);
+    update(prevF, F, prevSrc, prevCell, res);   // cell v-1 (for v == s0 a value never stored)
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       prevF[l] = F[l];
@@ -269,6 +287,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     }
     prevSrc = src;
   }
+  if (s1 > s0 + 1) put(s1 - 2, res);
   // top face of the segment
   double F[5];
   FaceRef fr = face_ref(s1);
@@ -286,7 +305,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     riemann<DIR>(prevR, eL, fr, F);
   }
   face_bg(s1, F);
-  emit(s1 - 1, prevF, F, prevSrc, prevCell);
+  update(prevF, F, prevSrc, prevCell, res);
+  put(s1 - 1, res);
 }
 
 // ---------------------------------------------------------------------------

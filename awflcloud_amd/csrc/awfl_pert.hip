@@ -156,8 +156,12 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     int src;
     bool zn = false;
     if (per) {
-      src = a % n;
-      if (src < 0) src += n;
+      if (n > hs) {                    // the walk reaches at most hs + 1 cells past either end
+        src = a < 0 ? a + n : (a >= n ? a - n : a);
+      } else {
+        src = a % n;
+        if (src < 0) src += n;
+      }
     } else if (a < 0) {
       src = 0; zn = true;
     } else if (a >= n) {
@@ -169,10 +173,12 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     float v = in[idx];                 // s; c enters only the update
     return (zn && l == NM) ? 0.0f : v;
   };
+  // the y sweep's background is its level's, loaded once
+  const PointBg bgl = DIR == 1 ? G.bgL[o] : PointBg{};
+  const FaceBgP fbl = DIR == 1 ? G.fbL[o] : FaceBgP{};
   auto bg_of = [&](int kc, CellBgP<NGLL>& bg) {
     if constexpr (DIR == 1) {
-      const PointBg b = G.bgL[o];
-      sfor<NGLL>([&](auto ii) { bg.pt[ii] = b; });
+      sfor<NGLL>([&](auto ii) { bg.pt[ii] = bgl; });
       bg.wall_bot = bg.wall_top = false;
     } else {
       sfor<NGLL>([&](auto ii) { bg.pt[ii] = G.bgG[kc * NGLL + ii]; });
@@ -180,16 +186,24 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
       bg.wall_top = !per && kc == n - 1;
     }
   };
-  auto face_bg = [&](int f) -> FaceBgP { return DIR == 1 ? G.fbL[o] : G.fbF[f]; };
-  auto emit = [&](int c, const float (&F0)[5], const float (&F1)[5], float src, const float (&sc)[5],
-                  const float (&cc)[5]) {
+  auto face_bg = [&](int f) -> FaceBgP { return DIR == 1 ? fbl : G.fbF[f]; };
+  // compensated update of one cell into registers, and its store
+  auto update = [&](const float (&F0)[5], const float (&F1)[5], float src, const float (&sc)[5],
+                    const float (&cc)[5], float (&so)[5], float (&co)[5]) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       float td = -(F1[l] - F0[l]) * K.rd;
       if (DIR == 2 && l == idW) td = src + td;
       if (DIR == 2 && G.sim2d && l == idV) td = 0.0f;
+      kahan(sc[l], cc[l], K.dt * td, so[l], co[l]);
+    }
+  };
+  auto put = [&](int c, const float (&so)[5], const float (&co)[5]) {
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
       const int64_t idx = l * G.sl + base + (int64_t)c * sa;
-      kahan(sc[l], cc[l], K.dt * td, out[idx], cout[idx]);
+      out[idx] = so[l];
+      cout[idx] = co[l];
     }
   };
 
@@ -203,13 +217,19 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     cell_edges_pert<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
   }
   // software pipeline: the stencil cell of the next iteration and the
-  // compensation of the next update are loaded one iteration ahead
-  float nxt[5], cnx[5], cprev[5];
+  // compensation of the next update are loaded one iteration ahead, and the
+  // updated cell of an iteration is stored at the top of the next one, ahead
+  // of that iteration's loads: a wait for the loads then never waits for
+  // stores issued just before it
+  float nxt[5], cnx[5], cprev[5], ps[5], pc[5];
   sfor<5>([&](auto l) {
     nxt[l] = ld(l, s0 + hs);
     cnx[l] = cin[l * G.sl + base + (int64_t)(((s0 - 1) % n + n) % n) * sa];
+    prevF[l] = 0.0f;
+    ps[l] = pc[l] = 0.0f;
   });
   for (int v = s0; v < s1; v++) {
+    if (v > s0 + 1) put(v - 2, ps, pc);
     // the window is centred on cell v-1, the cell updated in this iteration
     float prevS[5];
     sfor<5>([&](auto l) {
@@ -228,7 +248,8 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     const FaceBgP fb = face_bg(v);
     if (!per && v == 0) riemann_pert<DIR>(eL, eL, fb, F);
     else riemann_pert<DIR>(prevR, eL, fb, F);
-    if (v > s0) emit(v - 1, prevF, F, prevSrc, prevS, cprev);
+    // cell v-1 (for v == s0 a value that is never stored)
+    update(prevF, F, prevSrc, prevS, cprev, ps, pc);
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       prevF[l] = F[l];
@@ -236,6 +257,7 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     }
     prevSrc = src;
   }
+  if (s1 > s0 + 1) put(s1 - 2, ps, pc);
   float F[5], lastS[5];
   sfor<5>([&](auto l) { lastS[l] = w[l][hs]; });   // cell s1-1
   const FaceBgP fb = face_bg(s1);
@@ -252,7 +274,8 @@ __global__ __launch_bounds__(64) void k_sweep_col_f32(const float* __restrict__ 
     cell_edges_pert<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
     riemann_pert<DIR>(prevR, eL, fb, F);
   }
-  emit(s1 - 1, prevF, F, prevSrc, lastS, cnx);
+  update(prevF, F, prevSrc, lastS, cnx, ps, pc);
+  put(s1 - 1, ps, pc);
 }
 
 __global__ void k_to_f32(const double* __restrict__ d, float* __restrict__ s, float* __restrict__ c, int64_t n) {
