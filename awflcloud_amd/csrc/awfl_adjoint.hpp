@@ -453,23 +453,40 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   sfor<hs + 1>([&](auto i) { lo += tv[i]; });
   lo *= 1.0 / (hs + 1);
   tv[hs + 1] = lo + (tvhi - lo) * W::sigma;
+  // the NP reciprocals 1/(tv^2 + eps) and 1/(d^2 + w1 (1 - 2d)) each from
+  // one reciprocal of the product and the products of the others
   double rDn[NP], b[NP], w1[NP], m[NP], w[NP], rden[NP];
   double S1 = WENO_EPS;
-  sfor<NP>([&](auto i) {
-    constexpr double d = W::idl[i];
-    rDn[i] = rcp(tv[i] * tv[i] + WENO_EPS);
-    b[i] = d * rDn[i];
-    S1 += b[i];
-  });
+  {
+    double D[NP], ex[NP], P;
+    sfor<NP>([&](auto i) { D[i] = tv[i] * tv[i] + WENO_EPS; });
+    excl_products<NP>(D, ex, P);
+    const double rP = rcp(P);
+    sfor<NP>([&](auto i) {
+      constexpr double d = W::idl[i];
+      rDn[i] = ex[i] * rP;
+      b[i] = d * rDn[i];
+      S1 += b[i];
+    });
+  }
   const double R1 = rcp(S1);
   double S2 = WENO_EPS;
-  sfor<NP>([&](auto i) {
-    constexpr double d = W::idl[i];
-    w1[i] = b[i] * R1;
-    rden[i] = rcp(d * d + w1[i] * (1.0 - 2.0 * d));
-    m[i] = w1[i] * (d + d * d - 3.0 * d * w1[i] + w1[i] * w1[i]) * rden[i];
-    S2 += m[i];
-  });
+  {
+    double E[NP], ex[NP], P;
+    sfor<NP>([&](auto i) {
+      constexpr double d = W::idl[i];
+      w1[i] = b[i] * R1;
+      E[i] = d * d + w1[i] * (1.0 - 2.0 * d);
+    });
+    excl_products<NP>(E, ex, P);
+    const double rP = rcp(P);
+    sfor<NP>([&](auto i) {
+      constexpr double d = W::idl[i];
+      rden[i] = ex[i] * rP;
+      m[i] = w1[i] * (d + d * d - 3.0 * d * w1[i] + w1[i] * w1[i]) * rden[i];
+      S2 += m[i];
+    });
+  }
   const double R2 = rcp(S2);
   sfor<NP>([&](auto i) { w[i] = m[i] * R2; });
   // reverse: GLL projection, weighted sum
