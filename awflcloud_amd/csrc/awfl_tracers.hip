@@ -74,15 +74,16 @@ __device__ __forceinline__ int wrap_axis(int a, int n, bool per, bool& zn) {
 // One line of a DIR sweep with ntr tracers.  LDS (doubles), n cells, W =
 // n + 2hs, NF = n + 1 faces:
 //   state line [5][W] (then the state fluxes [5][NF]), mixing ratios
-//   [ntr][W], state edges [10][n], tracer edges [2 ntr][n] (then the
-//   limiter's cell factors), z sources [n], cell densities [n], tracer
-//   fluxes [ntr][NF], limiter direction [NF].
+//   [ntr][W] (then the tracer fluxes [ntr][NF]), state edges [10][n], tracer
+//   edges [2 ntr][n] (then the limiter's cell factors), cell densities [n],
+//   and for z the sources [n].  The limiter forms the direction of a face
+//   again from the state edges, which stay in place to the end.
 // Divisions by the density and by the edge densities are multiplications by
 // one reciprocal (rcp, ~1 ulp) shared by all tracers.
 template <int ORD, int NGLL, int DIR, bool TEND>
-__global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in, const double* __restrict__ tr_in,
-                                                  double* __restrict__ out, double* __restrict__ tr_out, Grid G,
-                                                  SweepK K, int ntr, unsigned pos) {
+__device__ __forceinline__ void sweep_tr(const double* __restrict__ in, const double* __restrict__ tr_in,
+                                         double* __restrict__ out, double* __restrict__ tr_out, const Grid& G,
+                                         const SweepK& K, int ntr, unsigned pos) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
   constexpr int NA = NGLL;
@@ -113,12 +114,11 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
   double* s_st = sm;                   // [5][W]
   double* s_F = sm;                    // [5][NF], over the state line once the cells are done
   double* s_tq = s_st + 5 * W;         // [ntr][W]
+  double* s_TF = s_tq;                 // [ntr][NF], over the mixing ratios once the cells are done
   double* s_e = s_tq + ntr * W;        // [10][n]
   double* s_te = s_e + 10 * n;         // [2 ntr][n]
-  double* s_src = s_te + 2 * ntr * n;  // [n]
-  double* s_dens = s_src + n;          // [n]
-  double* s_TF = s_dens + n;           // [ntr][NF]
-  double* s_ul = s_TF + ntr * NF;      // [NF]
+  double* s_dens = s_te + 2 * ntr * n; // [n]
+  double* s_src = s_dens + n;          // [n], z only
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   auto level = [&](int c) { return DIR == 2 ? c : kl; };
 
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
       s_e[l * n + c] = eL[l];
       s_e[(5 + l) * n + c] = eR[l];
     });
-    s_src[c] = sa;
+    if constexpr (DIR == 2) s_src[c] = sa;
     double rr0[NGLL];
     sfor<NGLL>([&](auto ii) { rr0[ii] = rcp(rl[0][ii]); });
     for (int t = 0; t < ntr; t++) {
@@ -262,7 +262,6 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
       const double tR = wr ? s_te[(2 * t + 1) * n + n - 1] : s_te[(2 * t) * n + b];
       s_TF[t * NF + f] = (un > 0 ? tL : tR) * up;
     }
-    s_ul[f] = 0.5 * (L[NM] + R[NM]);
   }
   __syncthreads();
 
@@ -277,12 +276,20 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
         const double f1 = fmin(s_TF[t * NF + c], 0.0);
         const double f2 = fmax(s_TF[t * NF + cp], 0.0);
         const double fluxOut = K.dt * (f2 - f1) * K.rd;
-        s_fac[t * n + c] = fmin(1.0, s_tq[t * W + c + hs] * s_dens[c] * rcp(fluxOut + TRACER_EPS));
+        // the cell's mixing ratio times its density, as staged in step 1
+        const double q = tr_in[t * G.sl + base + c * stride] * rcp(s_dens[c]);
+        s_fac[t * n + c] = fmin(1.0, q * s_dens[c] * rcp(fluxOut + TRACER_EPS));
       }
     }
     __syncthreads();
     for (int f = threadIdx.x; f < nfaces; f += blockDim.x) {
-      const double u = s_ul[f];
+      // the normal momentum average of the face's edge states (S:907)
+      const bool wl = !per && f == 0, wr = !per && f == n;
+      const int fa = wl ? 0 : (f == 0 ? n - 1 : f - 1);
+      const int fb = wr ? n - 1 : (f == n ? 0 : f);
+      const double uL = wl ? s_e[NM * n + 0] : s_e[(5 + NM) * n + fa];
+      const double uR = wr ? s_e[(5 + NM) * n + n - 1] : s_e[NM * n + fb];
+      const double u = 0.5 * (uL + uR);
       int a = -1;
       if (u > 0) a = f == 0 ? n - 1 : f - 1;
       else if (u < 0) a = f == n ? 0 : f;
@@ -317,6 +324,22 @@ __global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in,
       else tr_out[t * G.sl + o] = tr_in[t * G.sl + o] + K.dt * td;
     }
   }
+}
+
+// Entry points.  The z kernel carries a bound of three waves per SIMD: its
+// registers otherwise come to 171, two waves.  The x and y kernels stay
+// under 168 without it and were measured faster without it.
+template <int ORD, int NGLL, int DIR, bool TEND>
+__global__ __launch_bounds__(256) void k_sweep_tr(const double* __restrict__ in, const double* __restrict__ tr_in,
+                                                  double* __restrict__ out, double* __restrict__ tr_out, Grid G,
+                                                  SweepK K, int ntr, unsigned pos) {
+  sweep_tr<ORD, NGLL, DIR, TEND>(in, tr_in, out, tr_out, G, K, ntr, pos);
+}
+template <int ORD, int NGLL, bool TEND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_tr_z(
+    const double* __restrict__ in, const double* __restrict__ tr_in, double* __restrict__ out,
+    double* __restrict__ tr_out, Grid G, SweepK K, int ntr, unsigned pos) {
+  sweep_tr<ORD, NGLL, 2, TEND>(in, tr_in, out, tr_out, G, K, ntr, pos);
 }
 
 // tracers += dt * tracer_tend over the interior (Temporal_ader.h:67-69)
@@ -383,8 +406,8 @@ struct VariantTr {
 
 #define AWFL_VARIANT_TR(O, N)                                                                  \
   VariantTr {                                                                                  \
-    O, N, {k_sweep_tr<O, N, 0, false>, k_sweep_tr<O, N, 1, false>, k_sweep_tr<O, N, 2, false>}, \
-        {k_sweep_tr<O, N, 0, true>, k_sweep_tr<O, N, 1, true>, k_sweep_tr<O, N, 2, true>},     \
+    O, N, {k_sweep_tr<O, N, 0, false>, k_sweep_tr<O, N, 1, false>, k_sweep_tr_z<O, N, false>}, \
+        {k_sweep_tr<O, N, 0, true>, k_sweep_tr<O, N, 1, true>, k_sweep_tr_z<O, N, true>},       \
         k_init_tr<O>,                                                                          \
   }
 
@@ -407,7 +430,7 @@ int launch_sweep_tr(awfl_ctx* c, const double* in, const double* tr_in, double* 
   if (!v) return fail(c, AWFL_E_UNSUPPORTED, "tracers: (ord, ngll) not compiled in");
   const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
   const int W = n + 2 * c->hs, ntr = c->ntr;
-  const size_t lds = sizeof(double) * (size_t)(5 * W + ntr * W + 10 * n + 2 * ntr * n + 2 * n + (ntr + 1) * (n + 1));
+  const size_t lds = sizeof(double) * (size_t)(5 * W + ntr * W + 10 * n + 2 * ntr * n + (dir == 2 ? 2 : 1) * n);
   if (lds > 160 * 1024) return fail(c, AWFL_E_UNSUPPORTED, "tracers: line too long for the tracer sweep's LDS");
   KernelTr k = tend ? v->tend[dir] : v->upd[dir];
   if (lds > 64 * 1024)

@@ -256,11 +256,15 @@ def adjoint_leg(reps=3):
         tf = timed(lambda: [D.sweep(base, out, dt, d) for _ in range(10)]) / 10
         ta = timed(lambda: [D.sweep_adjoint(base, y, dt, d, out=yin) for _ in range(10)]) / 10
         sweep["xyz"[d]] = {"forward_ms": tf * 1e3, "adjoint_ms": ta * 1e3, "adjoint_over_forward": ta / tf}
+    # the adjoint of one whole step's three split sweeps against their forward
+    sf = sum(v["forward_ms"] for v in sweep.values())
+    sa = sum(v["adjoint_ms"] for v in sweep.values())
     return {"workload": "thermal bubble 128x128x64, WENO5/ADER3, FP64, %d steps, checkpoint every %d"
                         % (nsteps, every),
             "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3, "backward_over_forward": bwd / fwd,
             "note": "backward = segment recomputation + split-sweep adjoints",
-            "sweep": sweep}
+            "sweep": sweep,
+            "step_sweeps": {"forward_ms": sf, "adjoint_ms": sa, "adjoint_over_forward": sa / sf}}
 
 
 def main():
