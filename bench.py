@@ -27,9 +27,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_VALU_PEAK_TFS = 78.6    # MI355X FP64 vector
 BYTES_PER_CELL_SWEEP = 80.0  # read + write 5 doubles
-# FP64 flops per cell and sweep of WENO5/ADER3 counted from the reference loop
-# nests (FMA = 2, add/mul/div/sqrt/pow = 1): 779 FMA, 271 div, 5 pow, 1 sqrt.
-FLOPS_PER_CELL_SWEEP = 2412.0
+# FP64 flops per cell of each WENO5/ADER3 sweep (x, y, z) and of a whole step
+# (three sweeps and three tendency applications) at 256x256x128, counted by the
+# op-counting build of the oracle, which follows the reference's statements
+# (oracle/opcount.cpp; add/sub, mul, div, sqrt, pow = 1 each):
+# `oracle/opcount 256 256 128 5 3`.
+FLOPS_PER_CELL_SWEEP = {0: 2488.3, 1: 2480.3, 2: 2553.6}
+FLOPS_PER_CELL_STEP = 7552.2
 
 NX, NY, NZ = 256, 256, 128
 # HBM bytes (read + written) per launch of each FP64 sweep kernel at
@@ -330,7 +334,8 @@ def main():
     kms = {d: sum(a.elapsed_time(b) for a, b in ev) / len(ev) for d, ev in events.items()}
     dom = max(kms, key=kms.get)
     achieved = BYTES_PER_CELL_SWEEP * ncell / (kms[dom] * 1e-3) / 1e9
-    flops = FLOPS_PER_CELL_SWEEP * ncell / (kms[dom] * 1e-3) / 1e12
+    flops = FLOPS_PER_CELL_SWEEP[dom] * ncell / (kms[dom] * 1e-3) / 1e12
+    step_flops = FLOPS_PER_CELL_STEP * value / 1e12
     mass1 = D.state_sums(state)[0]
     finite = bool(torch.isfinite(state).all())
 
@@ -354,7 +359,10 @@ def main():
                      "kernel": ["sweep_x", "sweep_y", "sweep_z"][dom],
                      "kernel_ms": {["x", "y", "z"][d]: kms[d] for d in kms}},
         "valu": {"achieved_tflops": flops, "peak_tflops": FP64_VALU_PEAK_TFS, "frac": flops / FP64_VALU_PEAK_TFS,
-                 "flops_per_cell_sweep": FLOPS_PER_CELL_SWEEP},
+                 "flops_per_cell_sweep": FLOPS_PER_CELL_SWEEP[dom],
+                 "step_tflops": step_flops, "step_frac": step_flops / FP64_VALU_PEAK_TFS,
+                 "flops_per_cell_step": FLOPS_PER_CELL_STEP,
+                 "flop_count": "reference operations per cell (oracle/opcount.cpp), FMA counted as 2"},
         "checks": {"finite": finite, "mass_drift_rel": abs(mass1 - mass0) / max(abs(mass0), 1.0)},
     }
     if not args.no_fp32:
