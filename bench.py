@@ -45,18 +45,20 @@ TRAFFIC_BYTES = {0: 681.9e6, 1: 751.7e6, 2: 732.9e6}
 NX4, NY4, NZ4, STEPS4 = 512, 512, 256, 200
 
 
-def cpu_baseline(nthreads, target_s=12.0):
-    """Oracle step rate on the 64x64x32 bubble with `nthreads` threads."""
+def cpu_baseline(nthreads, target_s=12.0, shape=(64, 64, 32), max_steps=100):
+    """Oracle step rate on the bubble (64x64x32 unless given) with `nthreads`
+    threads: one step to size the sample, then up to `max_steps` steps for
+    about `target_s` seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    nx, ny, nz = 64, 64, 32
+    nx, ny, nz = shape
     O = pyoracle.Oracle(nx, ny, nz, bc=(0, 0, 1), nthreads=nthreads)
     spec = pyoracle.make_spec([dict(shape=1, x0=10000.0, y0=10000.0, z0=2000.0, xr=2000.0, yr=2000.0,
                                     zr=2000.0, amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1)
     st = O.init_state(spec)
     dt = O.compute_time_step(st, 0.8)
     t1 = O.run_steps(st, dt, 1)
-    steps = max(1, min(100, int(target_s / max(t1, 1e-3))))
+    steps = max(1, min(max_steps, int(target_s / max(t1, 1e-3))))
     t = O.run_steps(st, dt, steps)
     return nx * ny * nz * steps / t, steps, t
 
@@ -388,6 +390,11 @@ def main():
         vN, nN, tN = cpu_baseline(nthr, args.cpu_seconds / 2)
         out["cpu_baseline"]["all_cores"] = {"value": vN, "threads": nthr, "host_cpus": ncores,
                                             "sample": "same workload, %d steps in %.1f s" % (nN, tN)}
+        # the headline workload itself on all cores: two steps of 256x256x128
+        v2, n2, t2 = cpu_baseline(nthr, 1e9, shape=(NX, NY, NZ), max_steps=2)
+        out["cpu_baseline"]["all_cores_256x256x128"] = {
+            "value": v2, "threads": nthr, "ms_per_step": t2 / n2 * 1e3,
+            "sample": "the headline bubble at 256x256x128, %d step(s) after one untimed step" % n2}
     print(json.dumps(out))
 
 
