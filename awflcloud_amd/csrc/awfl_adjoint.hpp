@@ -421,10 +421,14 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   using W = Weno<ORD>;
   constexpr int hs = W::hs;
   constexpr int NP = hs + 2;
-  // primal: candidates, bridge, total variations, weights
+  // primal: candidates, bridge, total variations, weights.  As in the
+  // forward (reconstruct), the constant coefficients are never formed: the
+  // GLL values are u_hs + sum_{c >= 1} kAvgProj[c][.] aw[c].
   double a[NP][ORD];
   sfor<hs + 1>([&](auto i) {
+    a[i][0] = 0.0;
     sfor<hs + 1>([&](auto ii) {
+      if constexpr (ii == 0) return;
       double t = 0.0;
       sfor<hs + 1>([&](auto s) {
         constexpr double cf = W::recon[i][s][ii];
@@ -434,7 +438,9 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
     });
     sfor<ORD - hs - 1>([&](auto m) { a[i][hs + 1 + m] = 0.0; });
   });
+  a[hs + 1][0] = 0.0;
   sfor<ORD>([&](auto ii) {
+    if constexpr (ii == 0) return;
     double t = 0.0;
     sfor<ORD>([&](auto s) {
       constexpr double cf = kBridge<ORD>.m[s][ii];
@@ -489,12 +495,16 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   }
   const double R2 = rcp(S2);
   sfor<NP>([&](auto i) { w[i] = m[i] * R2; });
-  // reverse: GLL projection, weighted sum
+  // reverse: GLL projection (the centre cell's average takes the sum of the
+  // GLL adjoints), weighted sum
   double awb[ORD];
+  awb[0] = 0.0;
+  sfor<NGLL>([&](auto ii) { ub[hs] += gb[ii]; });
   sfor<ORD>([&](auto s) {
+    if constexpr (s == 0) return;
     double t = 0.0;
     sfor<NGLL>([&](auto ii) {
-      constexpr double cf = P::c2g[s][ii];
+      constexpr double cf = kAvgProj<ORD, NGLL>.m[s][ii];
       if constexpr (cf != 0) t += cf * gb[ii];
     });
     awb[s] = t;
@@ -502,8 +512,10 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   double ab[NP][ORD], wb[NP];
   sfor<NP>([&](auto i) {
     double t = 0.0;
+    ab[i][0] = 0.0;
     sfor<ORD>([&](auto ii) {
-      if constexpr (i == hs + 1 || ii <= hs) {
+      if constexpr (ii == 0) {
+      } else if constexpr (i == hs + 1 || ii <= hs) {
         t += awb[ii] * a[i][ii];
         ab[i][ii] = w[i] * awb[ii];
       } else {
@@ -545,9 +557,11 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
     sfor<hs + 1>([&](auto ii) { ab[i][ii] += aib[ii]; });
   });
   tv_adj<ORD>(a[hs + 1], tvhib, ab[hs + 1]);
-  // candidate and bridge polynomials
+  // candidate and bridge polynomials (their constant coefficients, never
+  // formed, pass nothing back)
   sfor<hs + 1>([&](auto i) {
     sfor<hs + 1>([&](auto ii) {
+      if constexpr (ii == 0) return;
       sfor<hs + 1>([&](auto s) {
         constexpr double cf = W::recon[i][s][ii];
         if constexpr (cf != 0) ub[i + s] += cf * ab[i][ii];
@@ -555,6 +569,7 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
     });
   });
   sfor<ORD>([&](auto ii) {
+    if constexpr (ii == 0) return;
     sfor<ORD>([&](auto s) {
       constexpr double cf = kBridge<ORD>.m[s][ii];
       if constexpr (cf != 0) ub[s] += cf * ab[hs + 1][ii];

@@ -109,10 +109,37 @@ struct BridgeMap {
 template <int ORD>
 inline constexpr BridgeMap<ORD> kBridge{};
 
+// Every candidate polynomial, the full-stencil polynomial and so the bridge
+// and any convex combination of them have the centre cell's average over the
+// centre cell [-1/2, 1/2]: a0 + sum_{c even >= 2} a_c / ((c + 1) 2^c) = u_hs.
+// So a reconstruction's constant coefficient need not be formed: its GLL
+// values are u_hs + sum_{c >= 1} (x^c - [c even] / ((c + 1) 2^c)) a_c.  The
+// map m[c][ii] holds those coefficients (row 0 unused).
+template <int ORD, int NGLL>
+struct AvgProjMap {
+  double m[ORD][NGLL];
+  constexpr AvgProjMap() : m() {
+    using P = Proj<ORD, NGLL>;
+    for (int c = 0; c < ORD; c++) {
+      double k = 0.0;
+      if (c % 2 == 0) {
+        double p2 = 1.0;
+        for (int e = 0; e < c; e++) p2 *= 2.0;
+        k = 1.0 / ((c + 1) * p2);
+      }
+      for (int ii = 0; ii < NGLL; ii++) m[c][ii] = c == 0 ? 0.0 : P::c2g[c][ii] - P::c2g[0][ii] * k;
+    }
+  }
+};
+template <int ORD, int NGLL>
+inline constexpr AvgProjMap<ORD, NGLL> kAvgProj{};
+
 // ---------------------------------------------------------------------------
 // weno::compute_weno_coefs: ord cell averages u -> ord polynomial coefficients
 // ---------------------------------------------------------------------------
-template <int ORD, class T>
+// NOAVG: the constant coefficients (candidates, bridge and aw[0]) are not
+// formed (aw[0] is left unset); see AvgProjMap.
+template <int ORD, class T, bool NOAVG = false>
 __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[ORD]) {
   using W = Weno<ORD>;
   constexpr int hs = W::hs;
@@ -120,6 +147,7 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   // hs+1 degree-hs candidates on sub-stencils i..i+hs (WenoLimiter.h:110-116)
   sfor<hs + 1>([&](auto i) {
     sfor<hs + 1>([&](auto ii) {
+      if constexpr (NOAVG && ii == 0) return;
       T t = -0.0;
       sfor<hs + 1>([&](auto s) {
         constexpr T cf = (T)W::recon[i][s][ii];
@@ -131,6 +159,7 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   // "bridge" polynomial (:117-131): the full-stencil polynomial less the
   // weighted candidates, scaled, as one constant map of the stencil
   sfor<ORD>([&](auto ii) {
+    if constexpr (NOAVG && ii == 0) return;
     T t = -0.0;
     sfor<ORD>([&](auto s) {
       constexpr T cf = (T)kBridge<ORD>.m[s][ii];
@@ -228,6 +257,7 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   }
   // weighted sum (:163-171); candidates have no coefficients above hs
   sfor<ORD>([&](auto ii) {
+    if constexpr (NOAVG && ii == 0) return;
     T t = -0.0;
     sfor<hs + 2>([&](auto i) {
       if constexpr (i == hs + 1 || ii <= hs) t += w[i] * a[i][ii];
@@ -241,13 +271,17 @@ template <int ORD, int NGLL, class T>
 __host__ __device__ __forceinline__ void reconstruct(const T (&u)[ORD], bool doweno, T (&g)[NGLL]) {
   using P = Proj<ORD, NGLL>;
   if (doweno) {
+    // the weighted polynomial's GLL values from the centre cell's average and
+    // its non-constant coefficients (AvgProjMap; c2g[0][.] = 1)
     T aw[ORD];
-    weno_coefs<ORD>(u, aw);
+    weno_coefs<ORD, T, true>(u, aw);
     sfor<NGLL>([&](auto ii) {
-      T t = -0.0;
+      T t = u[(ORD - 1) / 2];
       sfor<ORD>([&](auto s) {
-        constexpr T cf = (T)P::c2g[s][ii];
-        if constexpr (cf != 0) t += cf * aw[s];
+        if constexpr (s > 0) {
+          constexpr T cf = (T)kAvgProj<ORD, NGLL>.m[s][ii];
+          if constexpr (cf != 0) t += cf * aw[s];
+        }
       });
       g[ii] = t;
     });
