@@ -48,16 +48,24 @@ enum { idR = 0, idU = 1, idV = 2, idW = 3, idT = 4 };
 // ---------------------------------------------------------------------------
 template <int N, int M, class T>
 __host__ __device__ __forceinline__ T tv_eval(const T (&a)[M]) {
+  // grouped by the first coefficient: sum_j a_j (sum_{terms (j,k)} c a_k)
   T r = 0;
-  sfor<Tv<N>::n>([&](auto t) {
-    constexpr int j = Tv<N>::j[t];
-    constexpr int k = Tv<N>::k[t];
-    constexpr T c = (T)Tv<N>::c[t];
-    T term;
-    if constexpr (j == k) term = c * (a[j] * a[j]);
-    else term = c * a[j] * a[k];
-    if constexpr (t == 0) r = term;
-    else r += term;
+  bool first = true;
+  sfor<N>([&](auto j) {
+    T inner = 0;
+    bool any = false;
+    sfor<Tv<N>::n>([&](auto t) {
+      if constexpr (Tv<N>::j[t] == j) {
+        constexpr T c = (T)Tv<N>::c[t];
+        constexpr int k = Tv<N>::k[t];
+        inner = any ? inner + c * a[k] : c * a[k];
+        any = true;
+      }
+    });
+    if (any) {
+      r = first ? a[j] * inner : r + a[j] * inner;
+      first = false;
+    }
   });
   return r;
 }
@@ -109,6 +117,45 @@ struct BridgeMap {
 template <int ORD>
 inline constexpr BridgeMap<ORD> kBridge{};
 
+// The bridge map is mirror-symmetric: column c (coefficient c) has
+// m[hs+k][c] = (-1)^c m[hs-k][c] (odd columns vanish at the centre), so it is
+// applied to the sums u[hs+k] + u[hs-k] (even c) or differences (odd c).
+// kind[c]: 1 symmetric, -1 antisymmetric, 0 neither (the plain map is used);
+// cm[c] the centre entry, cf[c][k] the coefficient of pair k.
+template <int ORD>
+struct BridgeMirror {
+  int kind[ORD];
+  double cm[ORD];
+  double cf[ORD][ORD];
+  constexpr BridgeMirror() : kind(), cm(), cf() {
+    constexpr int hs = (ORD - 1) / 2;
+    for (int c = 0; c < ORD; c++) {
+      double mx = 0.0;
+      for (int s = 0; s < ORD; s++) {
+        const double v = kBridge<ORD>.m[s][c];
+        mx = v > mx ? v : (-v > mx ? -v : mx);
+      }
+      const double tol = 1e-12 * mx;
+      bool sym = true, anti = true;
+      for (int k = 1; k <= hs; k++) {
+        const double p = kBridge<ORD>.m[hs + k][c], q = kBridge<ORD>.m[hs - k][c];
+        const double ds = p - q, da = p + q;
+        if (ds > tol || -ds > tol) sym = false;
+        if (da > tol || -da > tol) anti = false;
+      }
+      const double mid = kBridge<ORD>.m[hs][c];
+      if (mid > tol || -mid > tol) anti = false;
+      kind[c] = sym ? 1 : (anti ? -1 : 0);
+      cm[c] = sym ? mid : 0.0;
+      for (int k = 1; k <= hs; k++)
+        cf[c][k] = sym ? 0.5 * (kBridge<ORD>.m[hs + k][c] + kBridge<ORD>.m[hs - k][c])
+                       : 0.5 * (kBridge<ORD>.m[hs + k][c] - kBridge<ORD>.m[hs - k][c]);
+    }
+  }
+};
+template <int ORD>
+inline constexpr BridgeMirror<ORD> kBridgeMirror{};
+
 // Every candidate polynomial, the full-stencil polynomial and so the bridge
 // and any convex combination of them have the centre cell's average over the
 // centre cell [-1/2, 1/2]: a0 + sum_{c even >= 2} a_c / ((c + 1) 2^c) = u_hs.
@@ -158,13 +205,32 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   });
   // "bridge" polynomial (:117-131): the full-stencil polynomial less the
   // weighted candidates, scaled, as one constant map of the stencil
+  T usum[hs + 1], udif[hs + 1];
+  sfor<hs>([&](auto km) {
+    constexpr int k = km + 1;
+    usum[k] = u[hs + k] + u[hs - k];
+    udif[k] = u[hs + k] - u[hs - k];
+  });
   sfor<ORD>([&](auto ii) {
     if constexpr (NOAVG && ii == 0) return;
     T t = -0.0;
-    sfor<ORD>([&](auto s) {
-      constexpr T cf = (T)kBridge<ORD>.m[s][ii];
-      if constexpr (cf != 0) t += cf * u[s];
-    });
+    if constexpr (kBridgeMirror<ORD>.kind[ii] == 1) {
+      if constexpr (kBridgeMirror<ORD>.cm[ii] != 0) t = (T)kBridgeMirror<ORD>.cm[ii] * u[hs];
+      sfor<hs>([&](auto km) {
+        constexpr T cf = (T)kBridgeMirror<ORD>.cf[ii][km + 1];
+        if constexpr (cf != 0) t += cf * usum[km + 1];
+      });
+    } else if constexpr (kBridgeMirror<ORD>.kind[ii] == -1) {
+      sfor<hs>([&](auto km) {
+        constexpr T cf = (T)kBridgeMirror<ORD>.cf[ii][km + 1];
+        if constexpr (cf != 0) t += cf * udif[km + 1];
+      });
+    } else {
+      sfor<ORD>([&](auto s) {
+        constexpr T cf = (T)kBridge<ORD>.m[s][ii];
+        if constexpr (cf != 0) t += cf * u[s];
+      });
+    }
     a[hs + 1][ii] = t;
   });
   // total variation of every candidate (:133-143), sigma blend (:145-151)
