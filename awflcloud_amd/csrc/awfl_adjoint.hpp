@@ -303,10 +303,10 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const FaceR
   double p = C0 * pow_at(rt, fr.rt);
   double ir = rcp(r);
   double cs2 = GAMMA * p * ir;
-  double cs = sqrt(cs2);
+  double cs, i2c;
+  sqrt_halfrsqrt(cs2, cs, i2c);     // as the forward riemann
   double it = rcp(t);
-  double ics = rcp(cs);
-  double i2c = 0.5 * ics;
+  double ics = 2.0 * i2c;
   const bool up = un > 0;
   const double(&U)[5] = up ? L : R;
   double a = U[4] * it;

@@ -583,9 +583,9 @@ __host__ __device__ __forceinline__ void riemann(const double (&L)[5], const dou
   }
   double p = C0 * pow_gamma_near(r * t, fma(r, t, -fr.rt.b) * fr.rt.rb, fr.rt.h, fr.rt.l);
   double cs2 = GAMMA * p * rcp(r);
-  double cs = sqrt(cs2);
+  double cs, r2c;
+  sqrt_halfrsqrt(cs2, cs, r2c);
   double rt = rcp(t);
-  double r2c = 0.5 * rcp(cs);
   double w1, w2, w3;
   if (un > 0) {
     double a = L[4] * rt;

@@ -47,6 +47,29 @@ __host__ __device__ __forceinline__ double rcp(double x) {
   return r;
 }
 
+// sqrt(x) and 1/(2 sqrt(x)) for a positive normal x: two Goldschmidt steps from
+// the hardware reciprocal square root, then one Newton correction of the root
+// (the refinement the library sqrt uses, without its scaling for denormal,
+// zero and infinite arguments, which a squared sound speed never is)
+__host__ __device__ __forceinline__ void sqrt_halfrsqrt(double x, double& s, double& hr) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rsq(x);
+  double g = x * r, h = 0.5 * r;
+  double e = fma(-g, h, 0.5);
+  h = fma(h, e, h);
+  g = fma(g, e, g);
+  e = fma(-g, h, 0.5);
+  h = fma(h, e, h);
+  g = fma(g, e, g);
+  const double d = fma(-g, g, x);
+  s = fma(d, h, g);
+  hr = h;
+#else
+  s = std::sqrt(x);
+  hr = 0.5 / s;
+#endif
+}
+
 // ln 2 split so that k*LN2_HI is exact for |k| < 2^21
 constexpr double LN2_HI = 0x1.62e42fefa3800p-1;
 constexpr double LN2_LO = 0x1.ef35793c76730p-45;
