@@ -39,7 +39,7 @@ NX, NY, NZ = 256, 256, 128
 # HBM bytes (read + written) per launch of each FP64 sweep kernel at
 # 256x256x128, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes calibrated by a
 # copy kernel of known size (profiles/sweeps_fp64_256x256x128_hbm_traffic.md)
-TRAFFIC_BYTES = {0: 682.0e6, 1: 751.0e6, 2: 733.3e6}
+TRAFFIC_BYTES = {0: 682.0e6, 1: 751.1e6, 2: 732.8e6}
 # FP32 perturbation-form comparison: 512^2 x 256 bubble + noise + band-limited
 # theta' field, 200 steps FP64 vs FP32 from the same initial state
 NX4, NY4, NZ4, STEPS4 = 512, 512, 256, 200
