@@ -316,3 +316,36 @@ def test_thermal_256x256x128_steps_match_oracle():
             for l in range(5):
                 assert e[l] <= 3 * sens[l] + 1e-14
         del g, ref
+
+
+@pytest.mark.skipif(not __import__("os").environ.get("AWFL_LONG_TESTS"),
+                    reason="about two minutes (100 oracle steps at 256x256x128); set AWFL_LONG_TESTS=1")
+def test_thermal_256x256x128_100_steps():
+    """The headline workload at its full size for 100 steps against the oracle
+    (OpenMP) from the same bytes: every variable within 1e-10 of its
+    perturbation maximum after 100 steps, rho and rho*theta within 1e-12 of the
+    full field at every checked step, mass conserved to 1e-12."""
+    _need_gpu()
+    from awflcloud_amd import bubble_spec
+    D, O = _pair(256, 256, 128, nthreads=0)
+    st = D.init_state(bubble_spec(bubble_dicts(), noise_amp=1e-3, noise_seed=1))
+    host = st.cpu().numpy().copy()
+    dt = D.compute_time_step(st, 0.8)
+    O.set_dt_cache(dt)
+    O.dim_switch = 1
+    rbar, rtbar = _bg(O)
+    m0 = D.state_sums(st)[0]
+    for s in range(100):
+        D.time_step(st, dt)
+        O.time_step(host, dt)
+        if s + 1 in (1, 10, 50, 100):
+            g = D.interior(st).cpu().numpy()
+            ref = O.interior(host)
+            e, f = per_var_rel_err(g, ref), full_rel_err(g, ref, rbar, rtbar)
+            print("256x256x128 step", s + 1, "pert", ["%.2e" % v for v in e], "full", ["%.2e" % v for v in f],
+                  flush=True)
+            assert max(f[0], f[4]) <= 1e-12
+            del g, ref
+    assert max(e) <= 1e-10
+    m1 = D.state_sums(st)[0]
+    assert abs(m1 - m0) <= 1e-12 * float(D.interior(st)[0].abs().sum())
