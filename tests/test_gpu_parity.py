@@ -349,3 +349,35 @@ def test_thermal_256x256x128_100_steps():
     assert max(e) <= 1e-10
     m1 = D.state_sums(st)[0]
     assert abs(m1 - m0) <= 1e-12 * float(D.interior(st)[0].abs().sum())
+
+
+@pytest.mark.skipif(not __import__("os").environ.get("AWFL_LONG_TESTS"),
+                    reason="about two minutes (50 WENO9 oracle steps at 192x192x96); set AWFL_LONG_TESTS=1")
+def test_weno9_192x192x96_50_steps():
+    """The high-order workload at its full size: 192x192x96, WENO9/ADER5, two
+    opposite seeded bubbles, CFL 0.5, 50 steps on the GPU against the oracle
+    (OpenMP): rho and rho*theta within 1e-12 of the full field after the first
+    step, every variable within 1e-10 of its perturbation maximum after 50."""
+    _need_gpu()
+    from awflcloud_amd.workloads import two_bubbles_spec
+    D, O = _pair(192, 192, 96, ord=9, ngll=5, nthreads=0)
+    st = D.init_state(two_bubbles_spec(seed=3))
+    host = st.cpu().numpy().copy()
+    dt = D.compute_time_step(st, 0.5)
+    O.set_dt_cache(dt)
+    O.dim_switch = 1
+    D.dim_switch = 1
+    rbar, rtbar = _bg(O)
+    for s in range(50):
+        D.time_step(st, dt)
+        O.time_step(host, dt)
+        if s + 1 in (1, 10, 50):
+            g = D.interior(st).cpu().numpy()
+            ref = O.interior(host)
+            e, f = per_var_rel_err(g, ref), full_rel_err(g, ref, rbar, rtbar)
+            print("192x192x96 WENO9 step", s + 1, "pert", ["%.2e" % v for v in e], "full",
+                  ["%.2e" % v for v in f], flush=True)
+            if s == 0:
+                assert max(f[0], f[4]) <= 1e-12
+            del g, ref
+    assert max(e) <= 1e-10
