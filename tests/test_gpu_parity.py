@@ -18,6 +18,8 @@ Gates:
     that the HIP path differs from the oracle no more than the oracle differs
     from itself under a one-ulp change of its libm.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -318,7 +320,7 @@ def test_thermal_256x256x128_steps_match_oracle():
         del g, ref
 
 
-@pytest.mark.skipif(not __import__("os").environ.get("AWFL_LONG_TESTS"),
+@pytest.mark.skipif(not os.environ.get("AWFL_LONG_TESTS"),
                     reason="about two minutes (100 oracle steps at 256x256x128); set AWFL_LONG_TESTS=1")
 def test_thermal_256x256x128_100_steps():
     """The headline workload at its full size for 100 steps against the oracle
@@ -351,7 +353,7 @@ def test_thermal_256x256x128_100_steps():
     assert abs(m1 - m0) <= 1e-12 * float(D.interior(st)[0].abs().sum())
 
 
-@pytest.mark.skipif(not __import__("os").environ.get("AWFL_LONG_TESTS"),
+@pytest.mark.skipif(not os.environ.get("AWFL_LONG_TESTS"),
                     reason="about two minutes (50 WENO9 oracle steps at 192x192x96); set AWFL_LONG_TESTS=1")
 def test_weno9_192x192x96_50_steps():
     """The high-order workload at its full size: 192x192x96, WENO9/ADER5, two
