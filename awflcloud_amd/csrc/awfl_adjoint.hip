@@ -906,6 +906,7 @@ int split_dirs(const awfl_ctx* c, int ds, int (&dirs)[3]) {
 
 int ensure_states(awfl_ctx* c, int64_t nstates) {
   if (c->a_states_n >= nstates) return AWFL_OK;
+  c->a_rec_ckpt = nullptr;
   if (c->a_states) HIP_TRY(c, hipFree(c->a_states));
   c->a_states = nullptr;
   c->a_states_n = 0;
@@ -944,6 +945,7 @@ int reverse_steps(awfl_ctx* c, int n0, int n1, int ds0, double dt, double* adj) 
 // split inputs of steps [n0, n1) from the state before step n0
 int record_steps(awfl_ctx* c, const double* start, int n0, int n1, int ds0, double dt) {
   const int64_t el = c->state_elems();
+  c->a_rec_ckpt = nullptr;             // a_states is rewritten
   int dirs[3];
   const int nd = split_dirs(c, 1, dirs);
   int rc = ensure_states(c, (int64_t)(n1 - n0) * nd);
@@ -1017,13 +1019,36 @@ int awfl_forward_ckpt(awfl_ctx* c, double* d_state, double dt, int nsteps, int e
   if (!d_state || !d_ckpt) return fail(c, AWFL_E_ARG, "null buffer");
   if (nsteps < 0 || every < 1) return fail(c, AWFL_E_ARG, "nsteps >= 0 and every >= 1");
   const int64_t el = c->state_elems();
-  for (int n = 0; n < nsteps; n++) {
+  // the last segment runs as the split sweeps the adjoint would recompute,
+  // keeping their inputs (a_states) for awfl_adjoint_ckpt on the same
+  // checkpoints; the earlier segments run as fused steps
+  const int ds0 = c->dimSwitch ? 1 : 0;
+  const int nseg = (nsteps + every - 1) / every;
+  const int nlast = nsteps > 0 ? (nseg - 1) * every : 0;
+  for (int n = 0; n < nlast; n++) {
     if (n % every == 0)
       HIP_TRY(c, hipMemcpyAsync(d_ckpt + (int64_t)(n / every) * el, d_state, sizeof(double) * el,
                                 hipMemcpyDeviceToDevice, c->stream));
     int rc = awfl_time_step(c, d_state, dt);
     if (rc != AWFL_OK) return rc;
   }
+  if (nsteps == 0) return AWFL_OK;
+  HIP_TRY(c, hipMemcpyAsync(d_ckpt + (int64_t)(nseg - 1) * el, d_state, sizeof(double) * el,
+                            hipMemcpyDeviceToDevice, c->stream));
+  int rc = record_steps(c, d_state, nlast, nsteps, ds0, dt);
+  if (rc != AWFL_OK) return rc;
+  // the last sweep of the last step, from its recorded input into the state
+  int dirs[3];
+  const int nd = split_dirs(c, ((nsteps - 1) % 2 == 0) ? ds0 : !ds0, dirs);
+  const int64_t last = (int64_t)(nsteps - nlast) * nd - 1;
+  rc = awfl_sweep(c, c->a_states + last * el, d_state, dt, dirs[nd - 1]);
+  if (rc != AWFL_OK) return rc;
+  c->dimSwitch = (nsteps % 2 == 0) ? ds0 : !ds0;   // as nsteps fused steps leave it
+  c->a_rec_ckpt = d_ckpt;
+  c->a_rec_dt = dt;
+  c->a_rec_nsteps = nsteps;
+  c->a_rec_every = every;
+  c->a_rec_ds0 = ds0;
   return AWFL_OK;
 }
 
@@ -1037,10 +1062,17 @@ int awfl_adjoint_ckpt(awfl_ctx* c, const double* d_ckpt, double dt, int nsteps, 
   const int64_t el = c->state_elems();
   const int ds0 = dim_switch0 ? 1 : 0;
   const int nseg = (nsteps + every - 1) / every;
+  // the last segment's split inputs are still in place when the forward run
+  // on these checkpoints was awfl_forward_ckpt on this context with the same
+  // arguments and nothing has rewritten them since
+  const bool kept = c->a_rec_ckpt == d_ckpt && c->a_rec_dt == dt && c->a_rec_nsteps == nsteps &&
+                    c->a_rec_every == every && c->a_rec_ds0 == ds0;
   for (int m = nseg - 1; m >= 0; m--) {
     const int n0 = m * every, n1 = std::min(nsteps, n0 + every);
-    rc = record_steps(c, d_ckpt + (int64_t)m * el, n0, n1, ds0, dt);
-    if (rc != AWFL_OK) return rc;
+    if (!(kept && m == nseg - 1)) {
+      rc = record_steps(c, d_ckpt + (int64_t)m * el, n0, n1, ds0, dt);
+      if (rc != AWFL_OK) return rc;
+    }
     rc = reverse_steps(c, n0, n1, ds0, dt, d_adj);
     if (rc != AWFL_OK) return rc;
   }

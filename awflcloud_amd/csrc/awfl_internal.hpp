@@ -101,6 +101,11 @@ struct awfl_ctx {
   double* a_tmp = nullptr;
   double* a_states = nullptr;
   int64_t a_states_n = 0;
+  // a_states holds the split inputs of the last segment of the checkpointed
+  // forward run described here (a_rec_ckpt null: of nothing reusable)
+  const double* a_rec_ckpt = nullptr;
+  double a_rec_dt = 0;
+  int a_rec_nsteps = 0, a_rec_every = 0, a_rec_ds0 = 0;
   // alternative integrators (awfl_splitting.hip): two tendency-sized arrays
   double* r_tend[2] = {nullptr, nullptr};
   // tracers (awfl_tracers.hip): count, positivity bit mask, ping-pong arrays

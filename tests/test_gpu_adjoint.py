@@ -196,15 +196,40 @@ def test_checkpointed_adjoint_equals_stepwise():
     a = y.clone()
     for s_before, ds in reversed(states):
         D.time_step_adjoint(s_before, a, dt, ds)
+    ds_after = D.dim_switch
     D.dim_switch = 1
     s2 = st.clone()
     ckpt = D.forward_ckpt(s2, dt, nsteps, every)
+    # the last segment runs as split sweeps that keep their inputs: the same
+    # state bit for bit, and the dimSwitch the fused steps leave
     assert torch.equal(s2, s)
+    assert D.dim_switch == ds_after
     b = y.clone()
     D.adjoint_ckpt(ckpt, dt, nsteps, every, 1, b)
     err = float((D.interior(a) - D.interior(b)).abs().max() / D.interior(a).abs().max())
     print("checkpointed vs stepwise %.2e" % err, flush=True)
     assert err <= 1e-14
+    # a second adjoint run on the same checkpoints recomputes every segment
+    # (the first one overwrote the kept inputs): bit for bit the same
+    c = y.clone()
+    D.adjoint_ckpt(ckpt, dt, nsteps, every, 1, c)
+    assert torch.equal(b, c)
+    # one segment (nsteps < every): kept inputs reused twice, and invalidated
+    # by a step adjoint in between
+    D.dim_switch = 1
+    s3 = st.clone()
+    ckpt1 = D.forward_ckpt(s3, dt, 3, every)
+    d1, d2, d3 = y.clone(), y.clone(), y.clone()
+    D.adjoint_ckpt(ckpt1, dt, 3, every, 1, d1)
+    D.adjoint_ckpt(ckpt1, dt, 3, every, 1, d2)
+    D.time_step_adjoint(states[0][0], y.clone(), dt, 1)
+    D.adjoint_ckpt(ckpt1, dt, 3, every, 1, d3)
+    ref = y.clone()
+    for s_before, ds in reversed(states[:3]):
+        D.time_step_adjoint(s_before, ref, dt, ds)
+    assert torch.equal(d1, d2) and torch.equal(d1, d3)
+    err1 = float((D.interior(ref) - D.interior(d1)).abs().max() / D.interior(ref).abs().max())
+    assert err1 <= 1e-14
 
 
 def _c5_problem():
