@@ -76,6 +76,12 @@ def build_adapter_test():
          "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
 
 
+def build_probe():
+    """VALU issue-rate probe (FMA chains in FP32/FP64, rcp_f32), run on the GPU by hand."""
+    run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] + [os.path.join(CSRC, "issue_probe.hip"), "-o",
+                                                            os.path.join(PKG, "issue_probe")])
+
+
 def build_oracle():
     run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -85,6 +91,7 @@ def build_all():
     build_hip()
     build_driver()
     build_adapter_test()
+    build_probe()
     build_oracle()
 
 
