@@ -61,14 +61,22 @@ __device__ __forceinline__ unsigned tile_index() {
   return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-// tile_index() decoded into the (x, y, z) block coordinates of the grid
+// tile_index() decoded into the (x, y, z) block coordinates of the grid, x
+// fastest, then y (ZFIRST: then z), so that consecutive tiles differ first in
+// x and then along the chosen axis
+template <bool ZFIRST = false>
 __device__ __forceinline__ dim3 tile_coords() {
   unsigned t = tile_index();
   dim3 b;
   b.x = t % gridDim.x;
   t /= gridDim.x;
-  b.y = t % gridDim.y;
-  b.z = t / gridDim.y;
+  if constexpr (ZFIRST) {
+    b.z = t % gridDim.z;
+    b.y = t / gridDim.z;
+  } else {
+    b.y = t % gridDim.y;
+    b.z = t / gridDim.y;
+  }
   return b;
 }
 
