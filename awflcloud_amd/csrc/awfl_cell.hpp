@@ -397,6 +397,7 @@ struct SweepK {
   double rd;              // 1 / dsw
   double ndt[KMAX];       // -1 / (dsw (kt+1)), kt = 0..NA-2
   double tav[KMAX];       // dt^kt / (kt+1), kt = 1..NA-1 (tav[0] unused)
+  unsigned char* br;      // upwind branch per face (record / replay kernels only)
 };
 
 // ---------------------------------------------------------------------------
@@ -559,10 +560,12 @@ __host__ __device__ __forceinline__ void cell_edges(const double (&st)[5][ORD], 
 // normal velocity test is strict (u > 0).
 // ---------------------------------------------------------------------------
 // un_out receives the averaged normal velocity the flux upwinds with (the
-// tracer fluxes upwind with the same value, S:891-900).
-template <int DIR>
+// tracer fluxes upwind with the same value, S:891-900).  With FORCE the
+// upwind side of waves 1-3 is up_forced (true: left) instead of the sign
+// test: the flux with a frozen branch, for difference checks of the adjoint.
+template <int DIR, bool FORCE = false>
 __host__ __device__ __forceinline__ void riemann(const double (&L)[5], const double (&R)[5], const FaceRef& fr,
-                                        double (&F)[5], double& un_out) {
+                                        double (&F)[5], double& un_out, bool up_forced = false) {
   constexpr int NM = 1 + DIR;
   // tangential momentum indices in the order the reference uses them
   constexpr int T1 = DIR == 0 ? 2 : 1;
@@ -587,7 +590,7 @@ __host__ __device__ __forceinline__ void riemann(const double (&L)[5], const dou
   sqrt_halfrsqrt(cs2, cs, r2c);
   double rt = rcp(t);
   double w1, w2, w3;
-  if (un > 0) {
+  if (FORCE ? up_forced : un > 0) {
     double a = L[4] * rt;
     w1 = L[0] - a;
     w2 = L[T1] - ut1 * a;

@@ -49,7 +49,27 @@ __device__ __forceinline__ int64_t tend_index(const Grid& G, int l, int k, int j
 #define AWFL_COL_WAVES 1
 #endif
 
-template <int ORD, int NGLL, bool TEND>
+// BR: 0 production; 1 also records each face's upwind branch (un > 0) in
+// K.br; 2 takes the branch from K.br instead of the sign test (frozen-branch
+// forward for difference checks of the adjoint).  Branch bytes: x face f of
+// line (k, j) at (k ny + j)(nx+1) + f; y face f of (k, i) at (k (ny+1) + f) nx
+// + i; z face f of (j, i) at (f ny + j) nx + i.
+template <int DIR, int BR>
+__device__ __forceinline__ void face_flux(const double (&qL)[5], const double (&qR)[5], const FaceRef& fr,
+                                          double (&F)[5], unsigned char* bp) {
+  if constexpr (BR == 0) {
+    riemann<DIR>(qL, qR, fr, F);
+  } else if constexpr (BR == 1) {
+    double un;
+    riemann<DIR>(qL, qR, fr, F, un);
+    *bp = un > 0 ? 1 : 0;
+  } else {
+    double un;
+    riemann<DIR, true>(qL, qR, fr, F, un, *bp != 0);
+  }
+}
+
+template <int ORD, int NGLL, bool TEND, int BR = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVES))) void k_sweep_x(const double* __restrict__ in, double* __restrict__ out, Grid G,
                                                  SweepK K) {
   constexpr int hs = (ORD - 1) / 2;
@@ -129,7 +149,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVE
       qL[l] = aL ? s_L[l * nx + a] : s_R[l * nx + a];
       qR[l] = bR ? s_R[l * nx + b] : s_L[l * nx + b];
     }
-    riemann<0>(qL, qR, fr, F);
+    face_flux<0, BR>(qL, qR, fr, F, BR ? K.br + ((int64_t)k * G.ny + j) * (nx + 1) + f : nullptr);
 #pragma unroll
     for (int l = 0; l < 5; l++) s_F[l * (nx + 1) + f] = F[l];
   }
@@ -150,7 +170,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVE
 // ---------------------------------------------------------------------------
 // y / z sweeps: one lane per x-column walking a segment of its column
 // ---------------------------------------------------------------------------
-template <int ORD, int NGLL, int DIR, bool TEND>
+template <int ORD, int NGLL, int DIR, bool TEND, int BR = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAVES))) void k_sweep_col(const double* __restrict__ in, double* __restrict__ out, Grid G,
                                                   SweepK K, int seglen) {
   constexpr int hs = (ORD - 1) / 2;
@@ -215,6 +235,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     }
     return fr;
   };
+  auto branch = [&](int f) -> unsigned char* {
+    if constexpr (BR == 0) return nullptr;
+    return K.br + (DIR == 1 ? ((int64_t)o * (G.ny + 1) + f) * G.nx + i : ((int64_t)f * G.ny + o) * G.nx + i);
+  };
   auto face_bg = [&](int f, double (&F)[5]) {   // S:1628-1629
     if constexpr (DIR == 2) {
       if (f < G.nz) F[idW] -= G.hyPressureGLL[f * NGLL + 0];
@@ -275,8 +299,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, src);
     double F[5];
     FaceRef fr = face_ref(v);
-    if (!per && v == 0) riemann<DIR>(eL, eL, fr, F);   // wall: both sides from the first cell
-    else riemann<DIR>(prevR, eL, fr, F);
+    if (!per && v == 0) face_flux<DIR, BR>(eL, eL, fr, F, branch(v));   // wall: both sides from the first cell
+    else face_flux<DIR, BR>(prevR, eL, fr, F, branch(v));
     face_bg(v, F);
     update(prevF, F, prevSrc, prevCell, res);   // cell v-1 (for v == s0 a value never stored)
 #pragma unroll
@@ -292,7 +316,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
   double F[5];
   FaceRef fr = face_ref(s1);
   if (!per && s1 == n) {
-    riemann<DIR>(prevR, prevR, fr, F);            // wall: both sides from the last cell
+    face_flux<DIR, BR>(prevR, prevR, fr, F, branch(s1));   // wall: both sides from the last cell
   } else {
     sfor<5>([&](auto l) {
       sfor<ORD - 1>([&](auto s) { w[l][s] = w[l][s + 1]; });
@@ -302,7 +326,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     bg_of(s1 % n, bg);
     double eL[5], eR[5], sa_;
     cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
-    riemann<DIR>(prevR, eL, fr, F);
+    face_flux<DIR, BR>(prevR, eL, fr, F, branch(s1));
   }
   face_bg(s1, F);
   update(prevF, F, prevSrc, prevCell, res);
@@ -850,19 +874,20 @@ int awfl_state_sums(awfl_ctx* c, const double* d_state, double* h_out) {
 }  // extern "C"
 
 
-// launch one sweep; tend selects the tendency output (layout [5][nz][ny][nx])
-static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {
+using SweepXFn = void (*)(const double*, double*, Grid, SweepK);
+using SweepColFn = void (*)(const double*, double*, Grid, SweepK, int);
+
+// launch one sweep with the given kernels (x: kx; y, z: kcol)
+static int launch_sweep_with(awfl_ctx* c, const double* in, double* out, const SweepK& K, int dir, SweepXFn kx,
+                             SweepColFn kcol) {
   Grid G = c->grid();
-  SweepK K = sweep_constants(c, dt, dir);
-  const Variant* v = static_cast<const Variant*>(c->var);
   if (dir == 0) {
     int nx = c->cfg.nx;
     int threads = std::min(256, (nx + 63) / 64 * 64);
     size_t lds = sizeof(double) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx);
-    auto kern = v->sweep_x[tend ? 1 : 0];
     if (lds > 64 * 1024)
-      HIP_TRY(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    kern<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, out, G, K);
+      HIP_TRY(c, hipFuncSetAttribute((const void*)kx, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    kx<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, out, G, K);
     LAUNCH_CHECK(c);
     return AWFL_OK;
   }
@@ -875,10 +900,16 @@ static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, i
   int seglen = (n + nseg - 1) / nseg;
   nseg = (n + seglen - 1) / seglen;
   dim3 grid(bx, other, nseg);
-  auto kern = dir == 1 ? v->sweep_y[tend ? 1 : 0] : v->sweep_z[tend ? 1 : 0];
-  kern<<<grid, 64, 0, c->stream>>>(in, out, G, K, seglen);
+  kcol<<<grid, 64, 0, c->stream>>>(in, out, G, K, seglen);
   LAUNCH_CHECK(c);
   return AWFL_OK;
+}
+
+// launch one sweep; tend selects the tendency output (layout [5][nz][ny][nx])
+static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {
+  const Variant* v = static_cast<const Variant*>(c->var);
+  return launch_sweep_with(c, in, out, sweep_constants(c, dt, dir), dir, v->sweep_x[tend ? 1 : 0],
+                           dir == 1 ? v->sweep_y[tend ? 1 : 0] : v->sweep_z[tend ? 1 : 0]);
 }
 
 static int fill(awfl_ctx* c, double* p, int64_t n, double v) {
@@ -929,6 +960,39 @@ int awfl_apply_tendencies(awfl_ctx* c, double* d_state, const double* d_tend, do
   int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
   k_apply<<<blocks, 256, 0, c->stream>>>(d_state, d_tend, G, dt);
   LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
+int awfl_time_step_branches(awfl_ctx* c, double* d_state, double dt, unsigned char* d_branch, int mode) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state || !d_branch) return fail(c, AWFL_E_ARG, "null state or branch buffer");
+  if (mode != 1 && mode != 2) return fail(c, AWFL_E_ARG, "mode must be 1 (record) or 2 (replay)");
+  if (c->cfg.ord != 5 || c->cfg.ngll != 3)
+    return fail(c, AWFL_E_UNSUPPORTED, "branch record / replay is compiled for (ord, ngll) = (5, 3) only");
+  for (int b = 0; b < 2; b++)
+    if (!c->d_buf[b]) HIP_TRY(c, hipMalloc(&c->d_buf[b], sizeof(double) * c->state_elems()));
+  static const int fwd[3] = {0, 1, 2}, bwd[3] = {2, 1, 0};
+  const int64_t per_split = (int64_t)(c->cfg.nx + 1) * (c->cfg.ny + 1) * (c->cfg.nz + 1);
+  int dirs[3], slot[3], nd = 0;
+  for (int s = 0; s < 3; s++) {
+    int d = c->dimSwitch ? fwd[s] : bwd[s];
+    if (d == 1 && c->sim2d) continue;
+    slot[nd] = s;
+    dirs[nd++] = d;
+  }
+  const double* cur = d_state;
+  for (int s = 0; s < nd; s++) {
+    double* dst = (s == nd - 1) ? d_state : (cur == c->d_buf[0] ? c->d_buf[1] : c->d_buf[0]);
+    SweepK K = sweep_constants(c, dt, dirs[s]);
+    K.br = d_branch + per_split * slot[s];
+    int rc = mode == 1 ? launch_sweep_with(c, cur, dst, K, dirs[s], k_sweep_x<5, 3, false, 1>,
+                                           dirs[s] == 1 ? k_sweep_col<5, 3, 1, false, 1> : k_sweep_col<5, 3, 2, false, 1>)
+                       : launch_sweep_with(c, cur, dst, K, dirs[s], k_sweep_x<5, 3, false, 2>,
+                                           dirs[s] == 1 ? k_sweep_col<5, 3, 1, false, 2> : k_sweep_col<5, 3, 2, false, 2>);
+    if (rc != AWFL_OK) return rc;
+    cur = dst;
+  }
+  c->dimSwitch = !c->dimSwitch;
   return AWFL_OK;
 }
 

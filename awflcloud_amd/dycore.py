@@ -109,6 +109,7 @@ def lib():
         L.awfl_compute_tendencies.argtypes = [P, P, P, D, C.c_int]
         L.awfl_apply_tendencies.argtypes = [P, P, P, D, C.c_int]
         L.awfl_time_step.argtypes = [P, P, D]
+        L.awfl_time_step_branches.argtypes = [P, P, D, P, C.c_int]
         L.awfl_sweep.argtypes = [P, P, P, D, C.c_int]
         L.awfl_get_dim_switch.argtypes = [P]
         L.awfl_set_dim_switch.argtypes = [P, C.c_int]
@@ -250,6 +251,21 @@ class Dycore:
     def time_step(self, state, dt):
         self._chk_state(state)
         self._check(self.L.awfl_time_step(self.h, self._ptr(state), dt))
+
+    def new_branch_record(self):
+        """uint8 device buffer for the upwind branches of one step (three
+        splits of (nx+1)(ny+1)(nz+1) faces each)."""
+        n = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
+        return self.torch.zeros(n, dtype=self.torch.uint8, device="cuda")
+
+    def time_step_branches(self, state, dt, branches, mode):
+        """time_step recording (mode 1) or imposing (mode 2) the upwind branch
+        of every face (awfl_time_step_branches)."""
+        self._chk_state(state)
+        if branches.dtype != self.torch.uint8 or not branches.is_cuda or branches.numel() < \
+                3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1):
+            raise ValueError("branch record must be a uint8 device tensor from new_branch_record()")
+        self._check(self.L.awfl_time_step_branches(self.h, self._ptr(state), dt, self._ptr(branches), mode))
 
     def time_step_unfused(self, state, tend, dt):
         """Temporal_ader::timeStep through computeTendencies + applyTendencies."""

@@ -111,6 +111,16 @@ int awfl_apply_tendencies(awfl_ctx* ctx, double* d_state, const double* d_tend, 
  * kernel that reconstructs, evolves, fluxes and applies the tendency.  The
  * result is left in d_state.  Uses two context-owned state-sized buffers. */
 int awfl_time_step(awfl_ctx* ctx, double* d_state, double dt);
+/* awfl_time_step with the upwind branch of every face recorded (mode 1) or
+ * imposed (mode 2).  The Riemann flux takes waves 1-3 from the left state
+ * when the averaged normal velocity is > 0 (S:863, 1221, 1602); mode 1 runs
+ * the step as awfl_time_step does and stores that choice per face, mode 2
+ * uses the stored choice instead of the sign test, which makes the step a
+ * smooth map near the recorded run (for difference checks of the adjoint).
+ * d_branch holds 3 * (nx+1)(ny+1)(nz+1) bytes, one block per split in
+ * order.  Diagnostic, no counterpart in the reference; (ord, ngll) = (5, 3)
+ * only. */
+int awfl_time_step_branches(awfl_ctx* ctx, double* d_state, double dt, unsigned char* d_branch, int mode);
 /* One fused split sweep in direction dir (0 x, 1 y, 2 z): d_out = d_in + dt*tend.
  * d_in and d_out must not alias. */
 int awfl_sweep(awfl_ctx* ctx, const double* d_in, double* d_out, double dt, int dir);

@@ -259,6 +259,20 @@ static double initConstBVF_pressure(double t0, double bvf, double z) {
   double r = initConstBVF_density(t0, bvf, z);
   return C0 * pow(r * t, GAMMA);
 }
+/* Point values of the two background families at height z (for tests):
+ * family 0 constant theta (t0 = 300 K, S:378-383), 1 constant Brunt-Vaisala
+ * frequency (t0 = 300 K, bvf = 0.01 1/s, S:384-391). */
+void oracle_profile(int family, double z, double* th, double* rh, double* ph) {
+  if (family == 0) {
+    *th = 300;
+    *rh = initConstTheta_density(300, z);
+    *ph = initConstTheta_pressure(300, z);
+  } else {
+    *th = initConstBVF_pot_temp(300, 0.01, z);
+    *rh = initConstBVF_density(300, 0.01, z);
+    *ph = initConstBVF_pressure(300, 0.01, z);
+  }
+}
 static double ellipsoid_linear(double x, double y, double z, double x0, double y0, double z0, double xrad,
                                double yrad, double zrad, double amp) {
   double xn = (x - x0) / xrad, yn = (y - y0) / yrad, zn = (z - z0) / zrad;

@@ -117,6 +117,11 @@ typedef struct {
 } oracle_init_spec;
 void oracle_init_state(const oracle_ctx* c, const oracle_init_spec* s, double* state);
 
+/* Background point values at height z: family 0 constant theta
+ * (Profiles.h:8-20), 1 constant Brunt-Vaisala frequency (Profiles.h:28-47),
+ * both with t0 = 300 K, bvf = 0.01 1/s as initState uses them (S:378-391). */
+void oracle_profile(int family, double z, double* th, double* rh, double* ph);
+
 /* Sensitivity probe: seed != 0 moves every pow() of the step by one ulp
  * (direction from a hash of the argument); 0 restores plain pow(). */
 void oracle_set_pow_jitter(uint64_t seed);

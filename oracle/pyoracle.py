@@ -69,6 +69,7 @@ def lib():
         L.oracle_tendencies_dir_tr.argtypes = [P, D, D, D, D, C.c_double, C.c_int]
         L.oracle_time_step_tr.argtypes = [P, D, D, C.c_double]
         L.oracle_init_tracers.argtypes = [P, D]
+        L.oracle_profile.argtypes = [C.c_int, C.c_double, D, D, D]
         L.oracle_set_pow_jitter.argtypes = [C.c_uint64]
         L.oracle_uniform_pm1.restype = C.c_double
         L.oracle_uniform_pm1.argtypes = [C.c_uint64, C.c_uint64]
@@ -79,6 +80,14 @@ def lib():
 def _dp(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def profile(family, z):
+    """(theta, rho, p) of the background at height z: family 0 constant theta,
+    1 constant Brunt-Vaisala frequency (Profiles.h:8-47)."""
+    th, rh, ph = C.c_double(), C.c_double(), C.c_double()
+    lib().oracle_profile(family, z, C.byref(th), C.byref(rh), C.byref(ph))
+    return th.value, rh.value, ph.value
 
 
 def make_spec(bubbles=(), noise_amp=0.0, noise_seed=0, modes=None, u0=0.0, v0=0.0, y2d=0.0, perturb_amp=0.0,

@@ -242,31 +242,70 @@ def _c5_problem():
     return D, base, dt, mask
 
 
-def test_gradient_of_box_objective_against_differences():
-    """128x128x64 bubble, 60 steps, checkpoints every 10: dJ/dtheta'_0 by the
-    adjoint against central differences (eps = 1e-6 K) along 8 directions.
+def _branch_records(D, base, dt, nsteps, theta=None):
+    """the upwind branch of every face of every split of the run from base
+    (+ rho_bar theta), recorded step by step; returns (records, final state)"""
+    from awflcloud_amd import adjoint as A
+    recs = [D.new_branch_record() for _ in range(nsteps)]
+    s = (A.with_theta(D, base, theta) if theta is not None else base.clone()).contiguous()
+    D.dim_switch = 1
+    for n in range(nsteps):
+        D.time_step_branches(s, dt, recs[n], 1)
+    return recs, s
 
-    The forward is only piecewise smooth: the upwind flux switches between
-    the left and right state on the sign of the normal velocity (strict >),
-    and with zero initial winds many faces sit near u_n = 0.  Differences
-    across such a switch do not converge as eps shrinks (they stay at
-    1e-5 - 1e-4 relative for eps from 1e-4 to 1e-8), so the gate here is
-    1e-3; the exactness of the adjoint is established by the dot-product
-    tests against the tangent-linear map and by the tangent-versus-difference
-    test on states where the forward is smooth."""
+
+def _frozen_run(D, base, dt, recs, theta):
+    """the run from base + rho_bar theta with every face's upwind branch
+    imposed from recs (a smooth map near the recorded run)"""
+    from awflcloud_amd import adjoint as A
+    s = A.with_theta(D, base, theta).contiguous()
+    D.dim_switch = 1
+    for r in recs:
+        D.time_step_branches(s, dt, r, 2)
+    return s
+
+
+def _flips(a, b):
+    return sum(int((x != y).sum()) for x, y in zip(a, b))
+
+
+def test_gradient_of_box_objective_against_differences(figures):
+    """128x128x64 bubble, 60 steps, checkpoints every 10: dJ/dtheta'_0 by the
+    adjoint against central differences (eps = 1e-6 K) along 8 seeded random
+    directions.
+
+    The forward is only piecewise smooth: the Riemann flux takes waves 1-3
+    from the left or right state on the sign of the averaged normal velocity
+    (strict >, S:863, 1221, 1602), and with zero initial winds many faces sit
+    near u_n = 0, where a perturbation of eps flips the choice.  The adjoint
+    differentiates the branch the forward took.  So the differences are taken
+    of the forward with every face's branch frozen as the base run took it
+    (awfl_time_step_branches): that map is smooth and its derivative at the
+    base is what the adjoint computes; gate 1e-6.  The plain forward's
+    differences and the number of faces whose branch differs between its +eps
+    and -eps runs are reported beside them."""
     _need_gpu()
     from awflcloud_amd import adjoint as A
     D, base, dt, mask = _c5_problem()
     nsteps, every = 60, 10
     theta = torch.zeros((D.nz, D.ny, D.nx), dtype=torch.float64, device="cuda", requires_grad=True)
-    J = A.theta_box_objective(D, A.run(A.with_theta(D, base, theta), D, dt, nsteps, every), mask)
+    final = A.run(A.with_theta(D, base, theta), D, dt, nsteps, every)
+    J = A.theta_box_objective(D, final, mask)
     J.backward()
     grad = theta.grad.detach()
-    print("J = %.10e, |grad| = %.3e" % (float(J), float(grad.norm())), flush=True)
+    recs, s_rec = _branch_records(D, base, dt, nsteps)
+    # recording changes nothing, and replaying the recorded branches from the
+    # base state reproduces the run (the same statements in other kernels)
+    s_rep = _frozen_run(D, base, dt, recs, torch.zeros_like(grad))
+    e_rec = float((s_rec - final.detach()).abs().max() / final.detach().abs().max())
+    e_rep = float((s_rep - s_rec).abs().max() / s_rec.abs().max())
+    faces = sum(int(r.numel()) for r in recs)
+    figures("J = %.10e, |grad| = %.3e, %d face records; recording run vs forward %.1e, replay vs recording %.1e"
+            % (float(J), float(grad.norm()), faces, e_rec, e_rep))
+    assert e_rec <= 1e-14 and e_rep <= 1e-14
 
-    def J_of(th):
-        with torch.no_grad():
-            return float(A.theta_box_objective(D, A.run(A.with_theta(D, base, th), D, dt, nsteps, every), mask))
+    def J_of(st):
+        return float(A.theta_box_objective(D, st, mask))
 
     gen = torch.Generator(device="cuda")
     gen.manual_seed(2024)
@@ -274,12 +313,21 @@ def test_gradient_of_box_objective_against_differences():
     worst = 0.0
     for k in range(8):
         d = torch.randn((D.nz, D.ny, D.nx), generator=gen, dtype=torch.float64, device="cuda")
-        fd = (J_of(eps * d) - J_of(-eps * d)) / (2 * eps)
         ad = float((grad * d).sum())
-        rel = abs(fd - ad) / abs(ad)
+        with torch.no_grad():
+            fd_frozen = (J_of(_frozen_run(D, base, dt, recs, eps * d))
+                         - J_of(_frozen_run(D, base, dt, recs, -eps * d))) / (2 * eps)
+            rp, sp = _branch_records(D, base, dt, nsteps, eps * d)
+            rm, sm = _branch_records(D, base, dt, nsteps, -eps * d)
+            fd_plain = (J_of(sp) - J_of(sm)) / (2 * eps)
+        rel = abs(fd_frozen - ad) / abs(ad)
+        rel_plain = abs(fd_plain - ad) / abs(ad)
         worst = max(worst, rel)
-        print("direction %d: adjoint %.12e  fd %.12e  rel %.2e" % (k, ad, fd, rel), flush=True)
-    assert worst <= 1e-3
+        figures("direction %d: adjoint %.12e, frozen-branch fd %.12e (rel %.2e), plain fd rel %.2e, "
+                "faces flipped +eps vs -eps %d, +eps vs base %d, -eps vs base %d"
+                % (k, ad, fd_frozen, rel, rel_plain, _flips(rp, rm), _flips(rp, recs), _flips(rm, recs)))
+        del rp, rm
+    assert worst <= 1e-6
 
 
 def test_whole_run_dot_product():
@@ -312,11 +360,12 @@ def test_whole_run_dot_product():
     assert abs(lhs - rhs) <= 1e-9 * scale
 
 
-def test_difference_quotients_plateau():
+def test_difference_quotients_plateau(figures):
     """The c5 gradient along one per-cell random and one smooth direction,
-    against central differences for eps = 1e-3 ... 1e-8 (printed).  The
-    relative differences stay at 1e-7 - 1e-4 without converging, the
-    signature of the piecewise-smooth forward (upwind switch at u_n = 0)."""
+    against central differences for eps = 1e-3 ... 1e-8: of the plain forward,
+    which stay at 1e-7 - 1e-4 without converging (the upwind switch at u_n = 0
+    makes it piecewise smooth), and of the forward with the base run's upwind
+    branches frozen, which converge like eps^2 down to the rounding floor."""
     _need_gpu()
     from awflcloud_amd import adjoint as A
     import math
@@ -325,10 +374,15 @@ def test_difference_quotients_plateau():
     theta = torch.zeros((D.nz, D.ny, D.nx), dtype=torch.float64, device="cuda", requires_grad=True)
     A.theta_box_objective(D, A.run(A.with_theta(D, base, theta), D, dt, nsteps, every), mask).backward()
     g = theta.grad.detach()
+    recs, _ = _branch_records(D, base, dt, nsteps)
 
-    def J_of(th):
+    def J_plain(th):
         with torch.no_grad():
             return float(A.theta_box_objective(D, A.run(A.with_theta(D, base, th), D, dt, nsteps, every), mask))
+
+    def J_frozen(th):
+        with torch.no_grad():
+            return float(A.theta_box_objective(D, _frozen_run(D, base, dt, recs, th), mask))
 
     gen = torch.Generator(device="cuda")
     gen.manual_seed(2024)
@@ -340,8 +394,12 @@ def test_difference_quotients_plateau():
                 + 0.5 * torch.cos(2 * math.pi * (x - 4 * y) + 1.1))
     for name, d in (("random", d_rand), ("smooth", d_smooth)):
         ad = float((g * d).sum())
+        frozen = {}
         for eps in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8):
-            fd = (J_of(eps * d) - J_of(-eps * d)) / (2 * eps)
-            rel = abs(fd - ad) / abs(ad)
-            print("%s direction, eps %.0e: adjoint %.12e fd %.12e rel %.2e" % (name, eps, ad, fd, rel), flush=True)
+            fd = (J_plain(eps * d) - J_plain(-eps * d)) / (2 * eps)
+            ff = (J_frozen(eps * d) - J_frozen(-eps * d)) / (2 * eps)
+            rel, frozen[eps] = abs(fd - ad) / abs(ad), abs(ff - ad) / abs(ad)
+            figures("%s direction, eps %.0e: adjoint %.12e, plain fd rel %.2e, frozen-branch fd rel %.2e"
+                    % (name, eps, ad, rel, frozen[eps]))
             assert rel <= 1e-2
+        assert frozen[1e-5] <= 1e-6 and frozen[1e-6] <= 1e-6

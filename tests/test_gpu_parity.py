@@ -18,8 +18,6 @@ Gates:
     that the HIP path differs from the oracle no more than the oracle differs
     from itself under a one-ulp change of its libm.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -320,9 +318,7 @@ def test_thermal_256x256x128_steps_match_oracle():
         del g, ref
 
 
-@pytest.mark.skipif(not os.environ.get("AWFL_LONG_TESTS"),
-                    reason="about two minutes (100 oracle steps at 256x256x128); set AWFL_LONG_TESTS=1")
-def test_thermal_256x256x128_100_steps():
+def test_thermal_256x256x128_100_steps(figures):
     """The headline workload at its full size for 100 steps against the oracle
     (OpenMP) from the same bytes: every variable within 1e-10 of its
     perturbation maximum after 100 steps, rho and rho*theta within 1e-12 of the
@@ -344,18 +340,16 @@ def test_thermal_256x256x128_100_steps():
             g = D.interior(st).cpu().numpy()
             ref = O.interior(host)
             e, f = per_var_rel_err(g, ref), full_rel_err(g, ref, rbar, rtbar)
-            print("256x256x128 step", s + 1, "pert", ["%.2e" % v for v in e], "full", ["%.2e" % v for v in f],
-                  flush=True)
+            figures("step %d pert %s full %s" % (s + 1, ["%.2e" % v for v in e], ["%.2e" % v for v in f]))
             assert max(f[0], f[4]) <= 1e-12
             del g, ref
     assert max(e) <= 1e-10
     m1 = D.state_sums(st)[0]
+    figures("mass drift %.2e of the summed |rho'|" % (abs(m1 - m0) / float(D.interior(st)[0].abs().sum())))
     assert abs(m1 - m0) <= 1e-12 * float(D.interior(st)[0].abs().sum())
 
 
-@pytest.mark.skipif(not os.environ.get("AWFL_LONG_TESTS"),
-                    reason="about two minutes (50 WENO9 oracle steps at 192x192x96); set AWFL_LONG_TESTS=1")
-def test_weno9_192x192x96_50_steps():
+def test_weno9_192x192x96_50_steps(figures):
     """The high-order workload at its full size: 192x192x96, WENO9/ADER5, two
     opposite seeded bubbles, CFL 0.5, 50 steps on the GPU against the oracle
     (OpenMP): rho and rho*theta within 1e-12 of the full field after the first
@@ -377,8 +371,7 @@ def test_weno9_192x192x96_50_steps():
             g = D.interior(st).cpu().numpy()
             ref = O.interior(host)
             e, f = per_var_rel_err(g, ref), full_rel_err(g, ref, rbar, rtbar)
-            print("192x192x96 WENO9 step", s + 1, "pert", ["%.2e" % v for v in e], "full",
-                  ["%.2e" % v for v in f], flush=True)
+            figures("step %d pert %s full %s" % (s + 1, ["%.2e" % v for v in e], ["%.2e" % v for v in f]))
             if s == 0:
                 assert max(f[0], f[4]) <= 1e-12
             del g, ref
