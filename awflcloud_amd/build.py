@@ -50,6 +50,46 @@ def build_hip():
     os.replace(tmp, LIB)
 
 
+# Parity builds of the FP64 (5, 3) sweeps (test libraries: the C ABI of
+# awfl_dycore.hip only), each with some of the production arithmetic
+# switched back toward the reference's, to attribute the difference between
+# the HIP step and the CPU oracle (tests/test_gpu_parity_builds.py).  Masks:
+# AWFL_PAR_DIV 1, _POW 2, _LIBPOW 4, _BRIDGE 8, _ORDER 16 (awfl_fastmath.hpp).
+PARITY_BUILDS = {
+    "div": (1, False),          # IEEE division and sqrt for rcp() and the sound speed
+    "pow": (2, False),          # general pow_gamma in place of the near-background series
+    "contract": (0, False),     # no FMA contraction
+    "bridge": (8, False),       # WENO bridge polynomial by its three statements
+    "fast_all": (1 | 2 | 8, True),
+    "ref_libm": (16 | 1 | 4, True),   # reference operation order, IEEE /, sqrt, device pow()
+    "ref_pg": (16 | 1, True),         # reference operation order, IEEE /, sqrt, pow_gamma
+}
+
+
+def parity_lib(name):
+    return os.path.join(PKG, "libawfl_parity_%s.so" % name)
+
+
+def build_parity():
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    for name, (mask, nocontract) in PARITY_BUILDS.items():
+        obj = os.path.join(objdir, "parity_%s.o" % name)
+        cmd = [HIPCC] + HIP_FLAGS + ["-DAWFL_LEAN", "-DAWFL_PARITY=%d" % mask] + \
+            (["-ffp-contract=off"] if nocontract or name == "contract" else []) + \
+            ["-c", os.path.join(CSRC, "awfl_dycore.hip"), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((name, obj, subprocess.Popen(cmd)))
+    failed = [n for n, _, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError("hipcc failed for parity builds " + ", ".join(failed))
+    for name, obj, _ in procs:
+        tmp = parity_lib(name) + ".tmp"
+        run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", obj, "-o", tmp])
+        os.replace(tmp, parity_lib(name))
+
+
 def build_driver():
     """C++ driver over the header-only Spatial/Temporal adapter (include/awfl)."""
     src = os.path.join(PKG, "driver", "driver.cpp")
@@ -89,6 +129,7 @@ def build_oracle():
 def build_all():
     build_tables()
     build_hip()
+    build_parity()
     build_driver()
     build_adapter_test()
     build_probe()

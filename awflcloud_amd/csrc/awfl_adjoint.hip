@@ -1064,9 +1064,12 @@ int awfl_adjoint_ckpt(awfl_ctx* c, const double* d_ckpt, double dt, int nsteps, 
   const int nseg = (nsteps + every - 1) / every;
   // the last segment's split inputs are still in place when the forward run
   // on these checkpoints was awfl_forward_ckpt on this context with the same
-  // arguments and nothing has rewritten them since
+  // arguments, on the same stream, and nothing has rewritten them since.
+  // They serve one adjoint only: a later adjoint on a buffer at the same
+  // address (reused by the caller for other contents) recomputes.
   const bool kept = c->a_rec_ckpt == d_ckpt && c->a_rec_dt == dt && c->a_rec_nsteps == nsteps &&
                     c->a_rec_every == every && c->a_rec_ds0 == ds0;
+  c->a_rec_ckpt = nullptr;
   for (int m = nseg - 1; m >= 0; m--) {
     const int n0 = m * every, n1 = std::min(nsteps, n0 + every);
     if (!(kept && m == nseg - 1)) {

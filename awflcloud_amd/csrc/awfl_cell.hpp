@@ -205,6 +205,15 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
   });
   // "bridge" polynomial (:117-131): the full-stencil polynomial less the
   // weighted candidates, scaled, as one constant map of the stencil
+#if AWFL_PARITY & AWFL_PAR_BRIDGE
+  sfor<ORD>([&](auto ii) {
+    if constexpr (NOAVG && ii == 0) return;
+    T t = -0.0;
+    sfor<ORD>([&](auto s) { t += (T)W::recon[hs + 1][s][ii] * u[s]; });
+    if constexpr (ii <= hs) sfor<hs + 1>([&](auto i) { t -= (T)W::idl[i] * a[i][ii]; });
+    a[hs + 1][ii] = t / (T)W::idl[hs + 1];
+  });
+#else
   T usum[hs + 1], udif[hs + 1];
   sfor<hs>([&](auto km) {
     constexpr int k = km + 1;
@@ -233,6 +242,7 @@ __host__ __device__ __forceinline__ void weno_coefs(const T (&u)[ORD], T (&aw)[O
     }
     a[hs + 1][ii] = t;
   });
+#endif
   // total variation of every candidate (:133-143), sigma blend (:145-151)
   T tv[hs + 2];
   sfor<hs + 1>([&](auto i) { tv[i] = tv_eval<hs + 1>(a[i]); });
@@ -397,6 +407,7 @@ struct SweepK {
   double rd;              // 1 / dsw
   double ndt[KMAX];       // -1 / (dsw (kt+1)), kt = 0..NA-2
   double tav[KMAX];       // dt^kt / (kt+1), kt = 1..NA-1 (tav[0] unused)
+  double dsw;             // grid spacing along the sweep (reference-order parity builds)
   unsigned char* br;      // upwind branch per face (record / replay kernels only)
 };
 

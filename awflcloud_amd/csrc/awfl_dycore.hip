@@ -30,6 +30,28 @@
 #include "awfl_cell.hpp"
 #include "awfl_profiles.hpp"
 #include "awfl_internal.hpp"
+#if AWFL_PARITY & AWFL_PAR_ORDER
+#include "awfl_cell_ref.hpp"
+#endif
+
+// Cell body of the sweeps: the production numerics, or (reference-order
+// parity builds) the reference's operation order.
+namespace cellns {
+#if AWFL_PARITY & AWFL_PAR_ORDER
+using awfl::ref::cell_edges;
+#else
+using awfl::cell_edges;
+#endif
+}  // namespace cellns
+
+// Flux divergence -(F_high - F_low)/d of one cell along the sweep.
+__device__ __forceinline__ double flux_div(double F1, double F0, const awfl::SweepK& K) {
+#if AWFL_PARITY & AWFL_PAR_ORDER
+  return -(F1 - F0) / K.dsw;
+#else
+  return -(F1 - F0) * K.rd;
+#endif
+}
 
 using namespace awfl;
 
@@ -58,7 +80,12 @@ template <int DIR, int BR>
 __device__ __forceinline__ void face_flux(const double (&qL)[5], const double (&qR)[5], const FaceRef& fr,
                                           double (&F)[5], unsigned char* bp) {
   if constexpr (BR == 0) {
+#if AWFL_PARITY & AWFL_PAR_ORDER
+    (void)fr;
+    awfl::ref::riemann<DIR>(qL, qR, F);
+#else
     riemann<DIR>(qL, qR, fr, F);
+#endif
   } else if constexpr (BR == 1) {
     double un;
     riemann<DIR>(qL, qR, fr, F, un);
@@ -119,7 +146,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVE
     double st[5][ORD];
     sfor<5>([&](auto l) { sfor<ORD>([&](auto s) { st[l][s] = s_st[l * W + c + s]; }); });
     double eL[5], eR[5], sa;
-    cell_edges<ORD, NGLL, 0>(st, bg, G.weno_s, G.weno_w, K, eL, eR, sa);
+    cellns::cell_edges<ORD, NGLL, 0>(st, bg, G.weno_s, G.weno_w, K, eL, eR, sa);
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       s_L[l * nx + c] = eL[l];
@@ -159,7 +186,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVE
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       const int cp = (per && c + 1 == nx) ? 0 : c + 1;
-      double td = -(s_F[l * (nx + 1) + cp] - s_F[l * (nx + 1) + c]) * K.rd;
+      double td = flux_div(s_F[l * (nx + 1) + cp], s_F[l * (nx + 1) + c], K);
       if (G.sim2d && l == idV) td = 0.0;
       if constexpr (TEND) out[tend_index(G, l, k, j, c)] = td;
       else out[l * G.sl + row + c] = in[l * G.sl + row + c] + K.dt * td;
@@ -250,7 +277,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
                     double (&res)[5]) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
-      double td = -(F1[l] - F0[l]) * K.rd;
+      double td = flux_div(F1[l], F0[l], K);
       if (DIR == 2 && l == idW) td = src + td;
       if (DIR == 2 && G.sim2d && l == idV) td = 0.0;
       res[l] = TEND ? td : cell[l] + K.dt * td;
@@ -275,7 +302,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     CellBg<NGLL> bg;
     bg_of(((s0 - 1) % n + n) % n, bg);
     double eL[5], sa_;
-    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
+    cellns::cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, prevR, sa_);
   }
   // software pipeline: the stencil cell of the next iteration is loaded one
   // iteration ahead, and the result of an iteration is stored at the top of
@@ -296,7 +323,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     CellBg<NGLL> bg;
     bg_of(v, bg);
     double eL[5], eR[5], src;
-    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, src);
+    cellns::cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, src);
     double F[5];
     FaceRef fr = face_ref(v);
     if (!per && v == 0) face_flux<DIR, BR>(eL, eL, fr, F, branch(v));   // wall: both sides from the first cell
@@ -325,7 +352,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AWFL_COL_WAV
     CellBg<NGLL> bg;
     bg_of(s1 % n, bg);
     double eL[5], eR[5], sa_;
-    cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
+    cellns::cell_edges<ORD, NGLL, DIR>(w, bg, G.weno_s, G.weno_w, K, eL, eR, sa_);
     face_flux<DIR, BR>(prevR, eL, fr, F, branch(s1));
   }
   face_bg(s1, F);
@@ -576,7 +603,17 @@ struct Variant {
           {k_sweep_col<O, N, 2, false>, k_sweep_col<O, N, 2, true>},                              \
           k_init<O>}
 
+#ifdef AWFL_LEAN
+// parity builds: the FP64 (5, 3) sweeps only, no FP32 form in the library
+const Variant kVariants[] = {AWFL_VARIANT(5, 3)};
+}  // namespace
+namespace awfl_detail {
+bool has_f32_variant(int, int) { return false; }
+}  // namespace awfl_detail
+namespace {
+#else
 const Variant kVariants[] = {AWFL_VARIANT(3, 2), AWFL_VARIANT(5, 3), AWFL_VARIANT(9, 5)};
+#endif
 
 const Variant* find_variant(int ord, int ngll) {
   for (const Variant& v : kVariants)
@@ -723,8 +760,11 @@ int awfl_create(const awfl_config* cfg, awfl_ctx** out) {
   return AWFL_OK;
 }
 
+static void timing_free(awfl_ctx* c);
+
 void awfl_destroy(awfl_ctx* c) {
   if (!c) return;
+  timing_free(c);
   if (c->d_hy) (void)hipFree(c->d_hy);
   if (c->d_pref) (void)hipFree(c->d_pref);
   if (c->d_red) (void)hipFree(c->d_red);
@@ -755,6 +795,9 @@ int awfl_num_split(const awfl_ctx* c) { return c ? 3 : AWFL_E_ARG; }
 
 int awfl_set_stream(awfl_ctx* c, void* s) {
   if (!c) return AWFL_E_ARG;
+  // split inputs kept by awfl_forward_ckpt were written on the old stream:
+  // not reused from the new one
+  if ((hipStream_t)s != c->stream) c->a_rec_ckpt = nullptr;
   c->stream = (hipStream_t)s;
   return AWFL_OK;
 }
@@ -967,6 +1010,10 @@ int awfl_time_step_branches(awfl_ctx* c, double* d_state, double dt, unsigned ch
   if (!c) return AWFL_E_ARG;
   if (!d_state || !d_branch) return fail(c, AWFL_E_ARG, "null state or branch buffer");
   if (mode != 1 && mode != 2) return fail(c, AWFL_E_ARG, "mode must be 1 (record) or 2 (replay)");
+#ifdef AWFL_LEAN
+  (void)dt;
+  return fail(c, AWFL_E_UNSUPPORTED, "branch record / replay is not built into parity builds");
+#else
   if (c->cfg.ord != 5 || c->cfg.ngll != 3)
     return fail(c, AWFL_E_UNSUPPORTED, "branch record / replay is compiled for (ord, ngll) = (5, 3) only");
   for (int b = 0; b < 2; b++)
@@ -994,6 +1041,7 @@ int awfl_time_step_branches(awfl_ctx* c, double* d_state, double dt, unsigned ch
   }
   c->dimSwitch = !c->dimSwitch;
   return AWFL_OK;
+#endif
 }
 
 int awfl_time_step(awfl_ctx* c, double* d_state, double dt) {
@@ -1011,11 +1059,61 @@ int awfl_time_step(awfl_ctx* c, double* d_state, double dt) {
   const double* cur = d_state;
   for (int s = 0; s < nd; s++) {
     double* dst = (s == nd - 1) ? d_state : (cur == c->d_buf[0] ? c->d_buf[1] : c->d_buf[0]);
+    const bool timed = 2 * (size_t)c->t_n + 1 < c->t_ev.size();
+    if (timed) HIP_TRY(c, hipEventRecord(c->t_ev[2 * c->t_n], c->stream));
     int rc = launch_sweep(c, cur, dst, dt, dirs[s], false);
     if (rc != AWFL_OK) return rc;
+    if (timed) {
+      HIP_TRY(c, hipEventRecord(c->t_ev[2 * c->t_n + 1], c->stream));
+      c->t_dir[c->t_n++] = dirs[s];
+    }
     cur = dst;
   }
   c->dimSwitch = !c->dimSwitch;
+  return AWFL_OK;
+}
+
+static void timing_free(awfl_ctx* c) {
+  for (hipEvent_t e : c->t_ev) (void)hipEventDestroy(e);
+  c->t_ev.clear();
+  c->t_dir.clear();
+  c->t_n = 0;
+}
+
+int awfl_sweep_timing(awfl_ctx* c, int max_sweeps) {
+  if (!c) return AWFL_E_ARG;
+  if (max_sweeps < 0) return fail(c, AWFL_E_ARG, "max_sweeps must be >= 0");
+  timing_free(c);
+  if (max_sweeps == 0) return AWFL_OK;
+  c->t_ev.resize(2 * (size_t)max_sweeps);
+  c->t_dir.resize(max_sweeps);
+  for (size_t i = 0; i < c->t_ev.size(); i++) {
+    hipError_t e = hipEventCreate(&c->t_ev[i]);
+    if (e != hipSuccess) {
+      c->t_ev.resize(i);
+      timing_free(c);
+      return fail(c, AWFL_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+    }
+  }
+  return AWFL_OK;
+}
+
+int awfl_sweep_times(awfl_ctx* c, double ms[3], int64_t count[3]) {
+  if (!c || !ms || !count) return AWFL_E_ARG;
+  for (int d = 0; d < 3; d++) {
+    ms[d] = 0;
+    count[d] = 0;
+  }
+  if (c->t_n > 0) HIP_TRY(c, hipEventSynchronize(c->t_ev[2 * c->t_n - 1]));
+  for (int t = 0; t < c->t_n; t++) {
+    float m = 0;
+    HIP_TRY(c, hipEventElapsedTime(&m, c->t_ev[2 * t], c->t_ev[2 * t + 1]));
+    ms[c->t_dir[t]] += m;
+    count[c->t_dir[t]]++;
+  }
+  for (int d = 0; d < 3; d++)
+    if (count[d]) ms[d] /= (double)count[d];
+  c->t_n = 0;
   return AWFL_OK;
 }
 

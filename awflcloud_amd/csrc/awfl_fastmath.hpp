@@ -13,7 +13,24 @@
 //   arbitrary exponents that never occur here).
 //
 // The same source compiles for the host, where it is unit-tested.
+//
+// Parity builds (test builds of the FP64 sweeps that attribute their
+// differences from the CPU oracle; AWFL_PARITY is a bit mask, 0 in the
+// library): AWFL_PAR_DIV makes rcp() an IEEE division and sqrt_halfrsqrt()
+// the correctly rounded sqrt and a division; AWFL_PAR_POW makes
+// pow_gamma_near() the general pow_gamma(); AWFL_PAR_LIBPOW makes pow_gamma()
+// the device library's pow; AWFL_PAR_BRIDGE forms the WENO bridge polynomial
+// by its three statements instead of the folded map; AWFL_PAR_ORDER runs the
+// cell and face bodies in the reference's operation order (awfl_cell_ref.hpp).
 #pragma once
+#define AWFL_PAR_DIV 1
+#define AWFL_PAR_POW 2
+#define AWFL_PAR_LIBPOW 4
+#define AWFL_PAR_BRIDGE 8
+#define AWFL_PAR_ORDER 16
+#ifndef AWFL_PARITY
+#define AWFL_PARITY 0
+#endif
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #else
@@ -35,6 +52,9 @@ using std::fabs;
 constexpr double POW_GAMMA = 1.40027894002789400278940027894;
 
 __host__ __device__ __forceinline__ double rcp(double x) {
+#if AWFL_PARITY & AWFL_PAR_DIV
+  return 1.0 / x;
+#endif
 #if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(x);
 #else
@@ -52,7 +72,10 @@ __host__ __device__ __forceinline__ double rcp(double x) {
 // (the refinement the library sqrt uses, without its scaling for denormal,
 // zero and infinite arguments, which a squared sound speed never is)
 __host__ __device__ __forceinline__ void sqrt_halfrsqrt(double x, double& s, double& hr) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if AWFL_PARITY & AWFL_PAR_DIV
+  s = std::sqrt(x);
+  hr = 0.5 / s;
+#elif defined(__HIP_DEVICE_COMPILE__)
   const double r = __builtin_amdgcn_rsq(x);
   double g = x * r, h = 0.5 * r;
   double e = fma(-g, h, 0.5);
@@ -100,6 +123,9 @@ __host__ __device__ __forceinline__ double pow_gamma(double x) {
   // no contraction of a*b+c: every rounding is the one written, so the device
   // and host builds give the same result
 #pragma clang fp contract(off)
+#endif
+#if AWFL_PARITY & AWFL_PAR_LIBPOW
+  return std::pow(x, POW_GAMMA);
 #endif
   // x = m 2^e, m in [sqrt(1/2), sqrt(2))
   int e;
@@ -170,6 +196,9 @@ constexpr double NEAR_MAX = 0.0625;
 __host__ __device__ __attribute__((noinline)) inline double pow_gamma_far(double x) { return pow_gamma(x); }
 
 __host__ __device__ __forceinline__ double pow_gamma_near(double x, double e, double Bh, double Bl) {
+#if AWFL_PARITY & AWFL_PAR_POW
+  return pow_gamma_far(x);
+#endif
   if (fabs(e) <= NEAR_MAX) {
     double q = BINOM_GAMMA.c[12];
 #pragma unroll

@@ -121,6 +121,11 @@ struct awfl_ctx {
   unsigned tpos = 0;
   double* t_buf[2] = {nullptr, nullptr};
   int n_red = 0;
+  // sweep timing of awfl_time_step (awfl_sweep_timing): event pairs recorded
+  // on the stream around each sweep launch, with the sweep's direction
+  std::vector<hipEvent_t> t_ev;
+  std::vector<int> t_dir;
+  int t_n = 0;
   std::string err;
   Grid grid() const;
   int64_t state_elems() const {
@@ -156,6 +161,7 @@ inline awfl::SweepK sweep_constants(const awfl_ctx* c, double dt, int dir) {
   awfl::SweepK K{};
   double dsw = dir == 0 ? c->dx : dir == 1 ? c->dy : c->dz;
   K.dt = dt;
+  K.dsw = dsw;
   K.rd = 1.0 / dsw;
   double dtmult = dt;
   for (int kt = 0; kt < awfl::KMAX; kt++) {

@@ -85,57 +85,70 @@ def band_limited_modes(seed, n=64, kmin=4.0, kmax=16.0, rms=0.2):
     return m
 
 
-_lib = None
+_P, _D, _I64 = C.c_void_p, C.c_double, C.c_int64
+# C ABI of include/awfl_dycore.h: name -> (argument types, result type)
+_SIGNATURES = {
+    "awfl_create": ([C.POINTER(AwflConfig), C.POINTER(_P)], C.c_int),
+    "awfl_destroy": ([_P], None),
+    "awfl_last_error": ([_P], C.c_char_p),
+    "awfl_layout": ([_P, C.POINTER(C.c_int), C.POINTER(_I64), C.POINTER(_I64)], C.c_int),
+    "awfl_num_split": ([_P], C.c_int),
+    "awfl_set_stream": ([_P, _P], C.c_int),
+    "awfl_init_state": ([_P, _P, C.POINTER(InitSpec)], C.c_int),
+    "awfl_compute_time_step": ([_P, _P, _D, C.POINTER(_D)], C.c_int),
+    "awfl_set_time_step": ([_P, _D], C.c_int),
+    "awfl_compute_tendencies": ([_P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_apply_tendencies": ([_P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_time_step": ([_P, _P, _D], C.c_int),
+    "awfl_sweep_timing": ([_P, C.c_int], C.c_int),
+    "awfl_sweep_times": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
+    "awfl_time_step_branches": ([_P, _P, _D, _P, C.c_int], C.c_int),
+    "awfl_sweep": ([_P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_get_dim_switch": ([_P], C.c_int),
+    "awfl_set_dim_switch": ([_P, C.c_int], C.c_int),
+    "awfl_get_background": ([_P, C.c_int, C.POINTER(_D), _I64], C.c_int),
+    "awfl_state_sums": ([_P, _P, C.POINTER(_D)], C.c_int),
+    "awfl_eos_pressure": ([_P, _P, _P, _I64], C.c_int),
+    "awfl_state_to_f32": ([_P, _P, _P, _P], C.c_int),
+    "awfl_state_from_f32": ([_P, _P, _P, _P], C.c_int),
+    "awfl_sweep_f32": ([_P, _P, _P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_time_step_f32": ([_P, _P, _P, _D], C.c_int),
+    "awfl_time_step_strang": ([_P, _P, _D], C.c_int),
+    "awfl_time_step_ssprk3": ([_P, _P, _D], C.c_int),
+    "awfl_sweep_tangent": ([_P, _P, _P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_sweep_adjoint": ([_P, _P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_time_step_adjoint": ([_P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_forward_ckpt": ([_P, _P, _D, C.c_int, C.c_int, _P], C.c_int),
+    "awfl_adjoint_ckpt": ([_P, _P, _D, C.c_int, C.c_int, C.c_int, _P], C.c_int),
+    "awfl_set_tracers": ([_P, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "awfl_num_tracers": ([_P], C.c_int),
+    "awfl_init_tracers": ([_P, _P], C.c_int),
+    "awfl_compute_tendencies_tracers": ([_P, _P, _P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_apply_tracer_tendencies": ([_P, _P, _P, _D], C.c_int),
+    "awfl_sweep_tracers": ([_P, _P, _P, _P, _P, _D, C.c_int], C.c_int),
+    "awfl_time_step_tracers": ([_P, _P, _P, _D], C.c_int),
+}
+
+_libs = {}
 
 
-def lib():
-    """Load libawfl_dycore.so; raises if it has not been built."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("HIP library %s is missing; run `python -m awflcloud_amd.build`" % LIB_PATH)
-        L = C.CDLL(LIB_PATH)
-        P, D, I64 = C.c_void_p, C.c_double, C.c_int64
-        L.awfl_create.argtypes = [C.POINTER(AwflConfig), C.POINTER(P)]
-        L.awfl_destroy.argtypes = [P]
-        L.awfl_last_error.restype = C.c_char_p
-        L.awfl_last_error.argtypes = [P]
-        L.awfl_layout.argtypes = [P, C.POINTER(C.c_int), C.POINTER(I64), C.POINTER(I64)]
-        L.awfl_num_split.argtypes = [P]
-        L.awfl_set_stream.argtypes = [P, P]
-        L.awfl_init_state.argtypes = [P, P, C.POINTER(InitSpec)]
-        L.awfl_compute_time_step.argtypes = [P, P, D, C.POINTER(D)]
-        L.awfl_set_time_step.argtypes = [P, D]
-        L.awfl_compute_tendencies.argtypes = [P, P, P, D, C.c_int]
-        L.awfl_apply_tendencies.argtypes = [P, P, P, D, C.c_int]
-        L.awfl_time_step.argtypes = [P, P, D]
-        L.awfl_time_step_branches.argtypes = [P, P, D, P, C.c_int]
-        L.awfl_sweep.argtypes = [P, P, P, D, C.c_int]
-        L.awfl_get_dim_switch.argtypes = [P]
-        L.awfl_set_dim_switch.argtypes = [P, C.c_int]
-        L.awfl_get_background.argtypes = [P, C.c_int, C.POINTER(D), I64]
-        L.awfl_state_sums.argtypes = [P, P, C.POINTER(D)]
-        L.awfl_eos_pressure.argtypes = [P, P, P, I64]
-        L.awfl_state_to_f32.argtypes = [P, P, P, P]
-        L.awfl_state_from_f32.argtypes = [P, P, P, P]
-        L.awfl_sweep_f32.argtypes = [P, P, P, P, P, D, C.c_int]
-        L.awfl_time_step_f32.argtypes = [P, P, P, D]
-        L.awfl_time_step_strang.argtypes = [P, P, D]
-        L.awfl_time_step_ssprk3.argtypes = [P, P, D]
-        L.awfl_sweep_tangent.argtypes = [P, P, P, P, P, D, C.c_int]
-        L.awfl_sweep_adjoint.argtypes = [P, P, P, P, D, C.c_int]
-        L.awfl_time_step_adjoint.argtypes = [P, P, P, D, C.c_int]
-        L.awfl_forward_ckpt.argtypes = [P, P, D, C.c_int, C.c_int, P]
-        L.awfl_adjoint_ckpt.argtypes = [P, P, D, C.c_int, C.c_int, C.c_int, P]
-        L.awfl_set_tracers.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
-        L.awfl_num_tracers.argtypes = [P]
-        L.awfl_init_tracers.argtypes = [P, P]
-        L.awfl_compute_tendencies_tracers.argtypes = [P, P, P, P, P, D, C.c_int]
-        L.awfl_apply_tracer_tendencies.argtypes = [P, P, P, D]
-        L.awfl_sweep_tracers.argtypes = [P, P, P, P, P, D, C.c_int]
-        L.awfl_time_step_tracers.argtypes = [P, P, P, D]
-        _lib = L
-    return _lib
+def lib(path=None):
+    """Load libawfl_dycore.so, or with `path` one of the parity test builds of
+    its FP64 sweeps (awfl_dycore.hip alone, so the FP32, adjoint, tracer and
+    integrator entry points are absent there); raises if it is not built."""
+    path = path or LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError("HIP library %s is missing; run `python -m awflcloud_amd.build`" % path)
+        L = C.CDLL(path)
+        for name, (argtypes, restype) in _SIGNATURES.items():
+            if path != LIB_PATH and not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.argtypes = argtypes
+            f.restype = restype
+        _libs[path] = L
+    return _libs[path]
 
 
 class AwflError(RuntimeError):
@@ -146,10 +159,10 @@ class Dycore:
     """One Spatial/Temporal instance on the current HIP device."""
 
     def __init__(self, nx, ny, nz, xlen=20000.0, ylen=20000.0, zlen=10000.0, bc=(0, 0, 1), ord=5, ngll=3,
-                 weno_scalars=True, weno_winds=True, data_spec=1, precision=AWFL_FP64, stream=None):
+                 weno_scalars=True, weno_winds=True, data_spec=1, precision=AWFL_FP64, stream=None, library=None):
         import torch
         self.torch = torch
-        self.L = lib()
+        self.L = lib(library)
         cfg = AwflConfig(nx, ny, nz, xlen, ylen, zlen, bc[0], bc[1], bc[2], int(weno_scalars), int(weno_winds),
                          ord, ngll, data_spec, precision)
         h = C.c_void_p()
@@ -251,6 +264,19 @@ class Dycore:
     def time_step(self, state, dt):
         self._chk_state(state)
         self._check(self.L.awfl_time_step(self.h, self._ptr(state), dt))
+
+    def sweep_timing(self, max_sweeps):
+        """time the next max_sweeps sweeps of time_step with HIP events on
+        the library's stream (0 stops)"""
+        self._check(self.L.awfl_sweep_timing(self.h, int(max_sweeps)))
+
+    def sweep_times(self):
+        """(mean ms per direction x, y, z; counts) of the sweeps timed since
+        the last call"""
+        ms = (C.c_double * 3)()
+        n = (C.c_int64 * 3)()
+        self._check(self.L.awfl_sweep_times(self.h, ms, n))
+        return list(ms), list(n)
 
     def new_branch_record(self):
         """uint8 device buffer for the upwind branches of one step (three

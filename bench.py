@@ -7,11 +7,14 @@ bubble (2 K, r = 2 km at 10, 10, 2 km) plus seeded +-1e-3 K noise, dt from the
 CFL condition (0.8) computed once and cached.  One step = three split sweeps
 (x,y,z then z,y,x alternately), each one fused kernel.
 
-Prints ONE JSON line.  `value` is cell-updates/s = nx*ny*nz*steps / wall with
-the state resident in HBM.  `roofline` is for the slowest (dominant) sweep
-kernel: algorithmic HBM bytes per launch (read + write 5 FP64 variables per
-cell = 80 B/cell) divided by its mean duration from HIP events recorded on the
-library's stream around every launch inside the timed region.
+Prints ONE JSON line.  `value` is cell-updates/s = nx*ny*nz*steps / wall of
+awfl_time_step calls with the state resident in HBM.  `roofline` is for the
+slowest (dominant) sweep kernel, timed by HIP events that the library records
+on its stream around every sweep launch inside the timed region: its FP64
+vector rate by the reference's operations per cell against 78.6 TFLOP/s, or
+its algorithmic HBM bytes (read + write 5 FP64 variables per cell = 80 B/cell)
+against 8 TB/s, whichever fraction is larger (the sweeps are VALU-bound); the
+other figure and the executed-instruction issue fraction sit beside it.
 `cpu_baseline` is the FP64 CPU oracle (a C restatement of the reference step)
 timed on this host on a bounded sample.
 """
@@ -35,6 +38,13 @@ BYTES_PER_CELL_SWEEP = 80.0  # read + write 5 doubles
 FLOPS_PER_CELL_SWEEP = {0: 2488.3, 1: 2480.3, 2: 2553.6}
 FLOPS_PER_CELL_STEP = 7552.2
 
+# VALU wave-instructions x 64 per cell of each FP64 sweep kernel (x, y, z) at
+# 256x256x128, from rocprofv3 SQ_INSTS_VALU (profiles/sweeps_valu_counters.md);
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz, one FP64 wave64 VALU instruction per
+# 4 cycles per SIMD
+VALU_INSTS_PER_CELL = {0: 1505.0, 1: 1363.0, 2: 1436.0}
+N_SIMD, CLOCK_HZ = 1024, 2.4e9
+
 NX, NY, NZ = 256, 256, 128
 # HBM bytes (read + written) per launch of each FP64 sweep kernel at
 # 256x256x128, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes calibrated by a
@@ -45,10 +55,10 @@ TRAFFIC_BYTES = {0: 682.0e6, 1: 751.1e6, 2: 732.8e6}
 NX4, NY4, NZ4, STEPS4 = 512, 512, 256, 200
 
 
-def cpu_baseline(nthreads, target_s=12.0, shape=(64, 64, 32), max_steps=100):
+def cpu_baseline(nthreads, target_s=12.0, shape=(64, 64, 32), max_steps=100, sized=True):
     """Oracle step rate on the bubble (64x64x32 unless given) with `nthreads`
     threads: one step to size the sample, then up to `max_steps` steps for
-    about `target_s` seconds."""
+    about `target_s` seconds (sized=False: just `max_steps` timed steps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     nx, ny, nz = shape
@@ -57,8 +67,11 @@ def cpu_baseline(nthreads, target_s=12.0, shape=(64, 64, 32), max_steps=100):
                                     zr=2000.0, amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1)
     st = O.init_state(spec)
     dt = O.compute_time_step(st, 0.8)
-    t1 = O.run_steps(st, dt, 1)
-    steps = max(1, min(max_steps, int(target_s / max(t1, 1e-3))))
+    if sized:
+        t1 = O.run_steps(st, dt, 1)
+        steps = max(1, min(max_steps, int(target_s / max(t1, 1e-3))))
+    else:
+        steps = max_steps
     t = O.run_steps(st, dt, steps)
     return nx * ny * nz * steps / t, steps, t
 
@@ -299,44 +312,33 @@ def main():
                              amp=2.0, pwr=1.0)], noise_amp=1e-3, noise_seed=1)
     state = D.init_state(spec)
     dt = D.compute_time_step(state, 0.8)
-    bufs = [torch.empty_like(state), torch.empty_like(state)]
     mass0 = D.state_sums(state)[0]
 
-    def step(ds, events=None):
-        """Temporal_ader::timeStep as three fused sweeps ping-ponging through
-        two buffers and ending in `state` (what awfl_time_step does)."""
-        dirs = (0, 1, 2) if ds else (2, 1, 0)
-        cur = state
-        for s, d in enumerate(dirs):
-            dst = state if s == 2 else (bufs[1] if cur is bufs[0] else bufs[0])
-            if events is not None:
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-                D.sweep(cur, dst, dt, d)
-                e1.record()
-                events[d].append((e0, e1))
-            else:
-                D.sweep(cur, dst, dt, d)
-            cur = dst
-        return not ds
-
-    ds = True
+    # Temporal_ader::timeStep through the C ABI's fused step (awfl_time_step);
+    # the library brackets every sweep launch with HIP events on its stream
     for _ in range(args.warmup):
-        ds = step(ds)
+        D.time_step(state, dt)
     torch.cuda.synchronize()
-    events = {0: [], 1: [], 2: []}
+    D.sweep_timing(3 * args.steps)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ds = step(ds, events)
+        D.time_step(state, dt)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    sweep_ms, sweep_n = D.sweep_times()
+    D.sweep_timing(0)
+    assert sweep_n == [args.steps] * 3, sweep_n
 
     ncell = NX * NY * NZ
     value = ncell * args.steps / wall
-    kms = {d: sum(a.elapsed_time(b) for a, b in ev) / len(ev) for d, ev in events.items()}
+    kms = {d: sweep_ms[d] for d in range(3)}
     dom = max(kms, key=kms.get)
     achieved = BYTES_PER_CELL_SWEEP * ncell / (kms[dom] * 1e-3) / 1e9
     flops = FLOPS_PER_CELL_SWEEP[dom] * ncell / (kms[dom] * 1e-3) / 1e12
+    # executed VALU wave-instructions per SIMD-cycle against the FP64 limit
+    # of one per 4 cycles, from the counted instructions per cell
+    issue = VALU_INSTS_PER_CELL[dom] * ncell / 64 / (kms[dom] * 1e-3) / (N_SIMD * CLOCK_HZ / 4)
+    hbm_frac, valu_frac = achieved / HBM_PEAK_GBS, flops / FP64_VALU_PEAK_TFS
     step_flops = FLOPS_PER_CELL_STEP * value / 1e12
     mass1 = D.state_sums(state)[0]
     finite = bool(torch.isfinite(state).all())
@@ -356,10 +358,20 @@ def main():
         "data": "synthetic",
         "config": {"workload": "thermal bubble 256x256x128, WENO5/ADER3, FP64, periodic x/y, wall z",
                    "nx": NX, "ny": NY, "nz": NZ, "ord": 5, "ngll": 3, "dt": dt, "precision": "fp64"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES[dom],
-                     "kernel": ["sweep_x", "sweep_y", "sweep_z"][dom],
-                     "kernel_ms": {["x", "y", "z"][d]: kms[d] for d in kms}},
+        # the binding roof is the larger fraction: the FP64 vector ALU (by the
+        # reference's operations per cell) rather than HBM
+        "roofline": ({"bound": "valu", "achieved": flops, "peak": FP64_VALU_PEAK_TFS, "unit": "TFLOP/s",
+                      "frac": valu_frac} if valu_frac >= hbm_frac else
+                     {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": hbm_frac}) | {
+            "traffic": TRAFFIC_BYTES[dom],
+            "hbm": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac,
+                    "bytes_per_cell": BYTES_PER_CELL_SWEEP},
+            "valu_issue_frac": issue,
+            "kernel": ["sweep_x", "sweep_y", "sweep_z"][dom],
+            "kernel_ms": {["x", "y", "z"][d]: kms[d] for d in kms},
+            "timing": "HIP events recorded by the library on its stream around every sweep launch of "
+                      "awfl_time_step in the timed region"},
         "valu": {"achieved_tflops": flops, "peak_tflops": FP64_VALU_PEAK_TFS, "frac": flops / FP64_VALU_PEAK_TFS,
                  "flops_per_cell_sweep": FLOPS_PER_CELL_SWEEP[dom],
                  "step_tflops": step_flops, "step_frac": step_flops / FP64_VALU_PEAK_TFS,
@@ -368,7 +380,6 @@ def main():
         "checks": {"finite": finite, "mass_drift_rel": abs(mass1 - mass0) / max(abs(mass0), 1.0)},
     }
     if not args.no_fp32:
-        del bufs
         torch.cuda.empty_cache()
         out["fp32_pert"] = fp32_leg(args.fp32_steps, 2)
     if not args.no_adjoint:
@@ -381,16 +392,20 @@ def main():
         torch.cuda.empty_cache()
         out["tracers"] = tracer_leg()
     if not args.no_cpu:
-        v1, n1, t1 = cpu_baseline(1, args.cpu_seconds)
+        # the headline workload itself, one step on one core (~16 s)
+        v1, n1, t1 = cpu_baseline(1, shape=(NX, NY, NZ), max_steps=1, sized=False)
         out["cpu_baseline"] = {"value": v1, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-                               "sample": "CPU oracle (C restatement, -O3, no FMA), 64x64x32 bubble, "
-                                         "WENO5/ADER3, %d steps in %.1f s, 1 thread" % (n1, t1)}
+                               "sample": "CPU oracle (C restatement, -O3, no FMA), the 256x256x128 bubble "
+                                         "above, WENO5/ADER3, its first step (%.1f s), 1 thread" % t1}
         ncores = len(os.sched_getaffinity(0))
         nthr = min(ncores, 16)
+        vs, ns, ts = cpu_baseline(1, args.cpu_seconds / 2)
+        out["cpu_baseline"]["small_1_core"] = {"value": vs, "threads": 1,
+                                               "sample": "64x64x32 bubble, %d steps in %.1f s" % (ns, ts)}
         vN, nN, tN = cpu_baseline(nthr, args.cpu_seconds / 2)
         out["cpu_baseline"]["all_cores"] = {"value": vN, "threads": nthr, "host_cpus": ncores,
-                                            "sample": "same workload, %d steps in %.1f s" % (nN, tN)}
-        # the headline workload itself on all cores: two steps of 256x256x128
+                                            "sample": "64x64x32 bubble, %d steps in %.1f s" % (nN, tN)}
+        # the headline workload on all cores: two steps of 256x256x128
         v2, n2, t2 = cpu_baseline(nthr, 1e9, shape=(NX, NY, NZ), max_steps=2)
         out["cpu_baseline"]["all_cores_256x256x128"] = {
             "value": v2, "threads": nthr, "ms_per_step": t2 / n2 * 1e3,

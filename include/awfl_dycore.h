@@ -121,6 +121,15 @@ int awfl_time_step(awfl_ctx* ctx, double* d_state, double dt);
  * order.  Diagnostic, no counterpart in the reference; (ord, ngll) = (5, 3)
  * only. */
 int awfl_time_step_branches(awfl_ctx* ctx, double* d_state, double dt, unsigned char* d_branch, int mode);
+/* Diagnostic timing of awfl_time_step, no counterpart in the reference (the
+ * reference driver times whole steps with std::chrono, driver.cpp:49-54):
+ * with max_sweeps > 0, the next max_sweeps sweep launches of awfl_time_step
+ * are bracketed by HIP events recorded on the context's stream (creates the
+ * events; 0 frees them and stops).  awfl_sweep_times waits for the last
+ * recorded event and returns, per direction (x, y, z), the mean duration in
+ * ms and the number of timed sweeps since the previous call. */
+int awfl_sweep_timing(awfl_ctx* ctx, int max_sweeps);
+int awfl_sweep_times(awfl_ctx* ctx, double ms[3], int64_t count[3]);
 /* One fused split sweep in direction dir (0 x, 1 y, 2 z): d_out = d_in + dt*tend.
  * d_in and d_out must not alias. */
 int awfl_sweep(awfl_ctx* ctx, const double* d_in, double* d_out, double dt, int dir);
@@ -218,8 +227,9 @@ int awfl_forward_ckpt(awfl_ctx* ctx, double* d_state, double dt, int nsteps, int
  * the split inputs of the segment's steps are recomputed from its checkpoint
  * (device memory for every*3 states) and the split adjoints applied in
  * reverse order; the last segment's inputs are taken as awfl_forward_ckpt
- * left them when it ran on the same checkpoints and arguments (the
- * checkpoint contents must be unchanged since). */
+ * left them when it ran on the same checkpoints and arguments, on the same
+ * stream, immediately before (the checkpoint contents must be unchanged
+ * since).  They serve one adjoint: any later adjoint recomputes them. */
 int awfl_adjoint_ckpt(awfl_ctx* ctx, const double* d_ckpt, double dt, int nsteps, int every, int dim_switch0,
                       double* d_adj);
 

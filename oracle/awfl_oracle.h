@@ -125,6 +125,11 @@ void oracle_profile(int family, double z, double* th, double* rh, double* ph);
 /* Sensitivity probe: seed != 0 moves every pow() of the step by one ulp
  * (direction from a hash of the argument); 0 restores plain pow(). */
 void oracle_set_pow_jitter(uint64_t seed);
+/* 0: the C library's pow (default); 1: x^GAMMA by the double-double method of
+ * the HIP parity builds (attribution of their differences only) */
+void oracle_set_pow_mode(int mode);
+/* x^GAMMA as the step evaluates it under the current mode and jitter */
+double oracle_eos_pow(double x);
 
 /* Counter-based uniform in [-1,1): SplitMix64 of (seed, index). */
 double oracle_uniform_pm1(uint64_t seed, uint64_t idx);

@@ -71,6 +71,9 @@ def lib():
         L.oracle_init_tracers.argtypes = [P, D]
         L.oracle_profile.argtypes = [C.c_int, C.c_double, D, D, D]
         L.oracle_set_pow_jitter.argtypes = [C.c_uint64]
+        L.oracle_set_pow_mode.argtypes = [C.c_int]
+        L.oracle_eos_pow.restype = C.c_double
+        L.oracle_eos_pow.argtypes = [C.c_double]
         L.oracle_uniform_pm1.restype = C.c_double
         L.oracle_uniform_pm1.argtypes = [C.c_uint64, C.c_uint64]
         _lib = L
@@ -312,6 +315,17 @@ class Oracle:
 def set_pow_jitter(seed):
     """Sensitivity probe: move every pow() of the step by one ulp (0 = off)."""
     lib().oracle_set_pow_jitter(seed)
+
+
+def set_pow_mode(mode):
+    """0: the C library's pow (default); 1: (rho theta)^gamma by the
+    double-double method of the HIP parity builds (attribution runs only)."""
+    lib().oracle_set_pow_mode(mode)
+
+
+def eos_pow(x):
+    """x^gamma as the oracle's step evaluates it (current pow mode)."""
+    return lib().oracle_eos_pow(x)
 
 
 def uniform_pm1(seed, idx):

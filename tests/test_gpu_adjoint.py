@@ -214,8 +214,9 @@ def test_checkpointed_adjoint_equals_stepwise():
     c = y.clone()
     D.adjoint_ckpt(ckpt, dt, nsteps, every, 1, c)
     assert torch.equal(b, c)
-    # one segment (nsteps < every): kept inputs reused twice, and invalidated
-    # by a step adjoint in between
+    # one segment (nsteps < every): the kept inputs serve the first adjoint,
+    # the second recomputes them, a step adjoint in between rewrites them:
+    # all three bit for bit the same
     D.dim_switch = 1
     s3 = st.clone()
     ckpt1 = D.forward_ckpt(s3, dt, 3, every)

@@ -51,9 +51,17 @@ def test_initial_data_matches_oracle(name, n, figures):
     if name == "igw":
         assert np.abs(ref[1]).max() > 0 and (n[1] == 1 or np.abs(ref[2]).max() > 0)
     err = per_var_rel_err(got, ref)
-    figures("per-variable error %s" % ["%.1e" % e for e in err])
-    # device sqrt/pow/cos/sin against glibc: a few ulp of each cell's integrand
-    assert max(err) <= 1e-13
+    hs = O.hs
+    rbar = O.background(0)[hs:hs + n[2]][:, None, None]
+    rtbar = O.background(3)[hs:hs + n[2]][:, None, None]
+    full = full_rel_err(got, ref, rbar, rtbar)
+    figures("per-variable error %s, of the full field %s" % (["%.1e" % e for e in err], ["%.1e" % e for e in full]))
+    # device sqrt/pow/cos/sin/exp against glibc: a few ulp of each cell's
+    # integrand rho_h (theta_bar + theta') - rho_h theta_bar, i.e. of the full
+    # rho*theta (~360); the igw perturbation (0.01 K) is 2e-5 of that, so its
+    # own relative error shows those ulps magnified (1.9e-12 measured)
+    assert max(err[1:4]) <= 1e-13
+    assert full[4] <= 1e-15 and err[4] <= 1e-11
     assert np.all(got[0] == 0) and np.all(got[3] == 0)
 
 
@@ -83,9 +91,17 @@ def test_igw_steps_match_oracle(n, bc, figures):
         g = D.interior(st).cpu().numpy()
         ref = O.interior(host)
         f = full_rel_err(g, ref, rbar, rtbar)
+        # momenta against the momentum scale of the flow (the 20 m/s winds):
+        # rho*w starts at zero, so its own maximum after a step is tiny
+        mscale = max(np.abs(ref[l]).max() for l in (1, 2, 3))
+        fm = [np.abs(g[l] - ref[l]).max() / mscale for l in (1, 2, 3)]
         assert max(f[0], f[4]) <= 1e-12 * (s + 1), (s, f)
-        assert max(f[1:4]) <= 1e-11 * (s + 1), (s, f)
+        assert max(fm) <= 1e-13 * (s + 1), (s, fm)
     e = per_var_rel_err(g, ref)
-    figures("after 10 steps: full %s pert %s" % (["%.1e" % v for v in f], ["%.1e" % v for v in e]))
-    assert max(e) <= 1e-10
+    figures("after 10 steps: rho, rho*theta of the full field %.1e %.1e; momenta of the momentum scale %s; "
+            "each variable of its own maximum %s" % (f[0], f[4], ["%.1e" % v for v in fm], ["%.1e" % v for v in e]))
+    # rho', rho*w and (rho*theta)' are the waves' own (zero or 0.01 K at the
+    # start), so their errors are gated against the full fields above; the
+    # wind-carrying momenta also against their own maxima
+    assert max(e[1], e[2]) <= 1e-12
     assert bool(torch.isfinite(st).all())
