@@ -23,6 +23,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -96,7 +98,11 @@ __device__ __forceinline__ void face_flux(const double (&qL)[5], const double (&
   }
 }
 
-template <int ORD, int NGLL, bool TEND, int BR = 0>
+// INPLACE: in == out is allowed.  Every global read of the line happens in
+// the staging loop, before the first barrier; the face fluxes get LDS of
+// their own, so the update takes each cell's old value from the staged line
+// instead of reading it again.  (The line belongs to this work-group alone.)
+template <int ORD, int NGLL, bool TEND, int BR = 0, bool INPLACE = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVES))) void k_sweep_x(const double* __restrict__ in, double* __restrict__ out, Grid G,
                                                  SweepK K) {
   constexpr int hs = (ORD - 1) / 2;
@@ -109,7 +115,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVE
   double* s_st = sm;              // [5][W]   state of the line with boundary cells
   double* s_L = sm + 5 * W;       // [5][nx]  left edge states
   double* s_R = s_L + 5 * nx;     // [5][nx]  right edge states
-  double* s_F = sm;               // [5][nx+1] face fluxes, reuses s_st after the edges are formed
+  // [5][nx+1] face fluxes: over s_st once the edges are formed, or after s_R
+  double* s_F = INPLACE ? s_R + 5 * nx : sm;
   const int64_t row = (int64_t)(hs + k) * G.sz + (int64_t)(hs + j) * G.sy + hs;
   const bool per = G.bc[0] == AWFL_BC_PERIODIC;
 
@@ -189,6 +196,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AWFL_X_WAVE
       double td = flux_div(s_F[l * (nx + 1) + cp], s_F[l * (nx + 1) + c], K);
       if (G.sim2d && l == idV) td = 0.0;
       if constexpr (TEND) out[tend_index(G, l, k, j, c)] = td;
+      else if constexpr (INPLACE) out[l * G.sl + row + c] = s_st[l * W + hs + c] + K.dt * td;
       else out[l * G.sl + row + c] = in[l * G.sl + row + c] + K.dt * td;
     }
   }
@@ -588,9 +596,19 @@ __global__ void k_init(double* __restrict__ st, Grid G, InitDev S) {
 // ---------------------------------------------------------------------------
 // compiled (ord, ngll) variants
 // ---------------------------------------------------------------------------
+// the in-place x kernel of awfl_time_step_noalloc, built where the sweeps use
+// no scratch memory (the (9,5) kernels spill, and the runtime grows its
+// scratch pool on their first launch, an allocation the caller cannot avoid)
+template <int O, int N>
+constexpr void (*inplace_x())(const double*, double*, Grid, SweepK) {
+  if constexpr (O == 9) return nullptr;
+  else return k_sweep_x<O, N, false, 0, true>;
+}
+
 struct Variant {
   int ord, ngll;
   void (*sweep_x[2])(const double*, double*, Grid, SweepK);
+  void (*sweep_x_inplace)(const double*, double*, Grid, SweepK);
   void (*sweep_y[2])(const double*, double*, Grid, SweepK, int);
   void (*sweep_z[2])(const double*, double*, Grid, SweepK, int);
   void (*init)(double*, Grid, InitDev);
@@ -599,6 +617,7 @@ struct Variant {
 #define AWFL_VARIANT(O, N)                                                                        \
   Variant{O, N,                                                                                   \
           {k_sweep_x<O, N, false>, k_sweep_x<O, N, true>},                                        \
+          inplace_x<O, N>(),                                                                      \
           {k_sweep_col<O, N, 1, false>, k_sweep_col<O, N, 1, true>},                              \
           {k_sweep_col<O, N, 2, false>, k_sweep_col<O, N, 2, true>},                              \
           k_init<O>}
@@ -699,6 +718,43 @@ static std::vector<PowRef> power_references(const awfl_ctx* c) {
   return out;
 }
 
+// Scratch of awfl_time_step_noalloc: the one state-sized intermediate of a
+// step, in a file-scope device array so that the entry point allocates
+// nothing.  Each device holds its own copy from the library's first use on it
+// until the process ends.  Host code never uses the array's name as an
+// address: it reaches each device's copy through the address
+// hipGetSymbolAddress returns with that device current, obtained once per
+// device by awfl_create and kept in g_noalloc_ptr.  Nothing in it needs to
+// start at zero: every cell a sweep reads from it was written by the sweep
+// before (halo cells are never read).
+#ifdef AWFL_LEAN
+// (parity test builds carry no scratch: awfl_time_step_noalloc refuses)
+static __device__ __align__(256) double scratch[32];
+#else
+static __device__ __align__(256) double scratch[AWFL_NOALLOC_MAX_BYTES / sizeof(double)];
+#endif
+
+namespace {
+std::mutex g_noalloc_mu;
+std::map<int, double*> g_noalloc_ptr;
+
+// this device's scratch address (device current); error text on failure
+double* noalloc_scratch(int device, std::string& err) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  auto it = g_noalloc_ptr.find(device);
+  if (it != g_noalloc_ptr.end()) return it->second;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return nullptr;
+  }
+  g_noalloc_ptr[device] = static_cast<double*>(p);
+  return static_cast<double*>(p);
+}
+}  // namespace
+
 extern "C" {
 
 const char* awfl_last_error(const awfl_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
@@ -734,6 +790,7 @@ int awfl_create(const awfl_config* cfg, awfl_ctx** out) {
     delete c;
     return fail(nullptr, AWFL_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
   }
+  c->noalloc = noalloc_scratch(c->device, c->noalloc_err);
   build_background(c);
   int nzh = cfg->nz + 2 * hs, ng = cfg->nz * cfg->ngll;
   std::vector<double> packed;
@@ -921,13 +978,14 @@ using SweepXFn = void (*)(const double*, double*, Grid, SweepK);
 using SweepColFn = void (*)(const double*, double*, Grid, SweepK, int);
 
 // launch one sweep with the given kernels (x: kx; y, z: kcol)
+// (x_flux_lds: the in-place x kernel keeps its face fluxes apart from the line)
 static int launch_sweep_with(awfl_ctx* c, const double* in, double* out, const SweepK& K, int dir, SweepXFn kx,
-                             SweepColFn kcol) {
+                             SweepColFn kcol, bool x_flux_lds = false) {
   Grid G = c->grid();
   if (dir == 0) {
     int nx = c->cfg.nx;
     int threads = std::min(256, (nx + 63) / 64 * 64);
-    size_t lds = sizeof(double) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx);
+    size_t lds = sizeof(double) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx + (x_flux_lds ? 5 * (nx + 1) : 0));
     if (lds > 64 * 1024)
       HIP_TRY(c, hipFuncSetAttribute((const void*)kx, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     kx<<<c->cfg.ny * c->cfg.nz, threads, lds, c->stream>>>(in, out, G, K);
@@ -1042,6 +1100,62 @@ int awfl_time_step_branches(awfl_ctx* c, double* d_state, double dt, unsigned ch
   c->dimSwitch = !c->dimSwitch;
   return AWFL_OK;
 #endif
+}
+
+int awfl_time_step_noalloc(awfl_ctx* c, double* d_state, double dt) {
+  if (!c) return AWFL_E_ARG;
+  if (!d_state) return fail(c, AWFL_E_ARG, "null state");
+  const int64_t bytes = (int64_t)sizeof(double) * c->state_elems();
+#ifdef AWFL_LEAN
+  return fail(c, AWFL_E_UNSUPPORTED, "awfl_time_step_noalloc is not built into parity builds");
+#endif
+  if (bytes > (int64_t)AWFL_NOALLOC_MAX_BYTES)
+    return fail(c, AWFL_E_ARG, "awfl_time_step_noalloc: a state of " + std::to_string(bytes) +
+                                   " bytes exceeds its scratch of " + std::to_string((long long)AWFL_NOALLOC_MAX_BYTES) +
+                                   " bytes; use awfl_time_step");
+  const int64_t lds = (int64_t)sizeof(double) * (5 * (c->cfg.nx + 2 * c->hs) + 15 * c->cfg.nx + 5);
+  if (lds > 160 * 1024) return fail(c, AWFL_E_UNSUPPORTED, "awfl_time_step_noalloc: nx too large for the in-place x line");
+  const Variant* v = static_cast<const Variant*>(c->var);
+  if (!v->sweep_x_inplace)
+    return fail(c, AWFL_E_UNSUPPORTED, "awfl_time_step_noalloc: the (9, 5) sweeps spill to scratch memory, which the "
+                                       "HIP runtime allocates on their first launch; use awfl_time_step");
+  if (!c->noalloc) return fail(c, AWFL_E_HIP, "awfl_time_step_noalloc: no scratch address: " + c->noalloc_err);
+  int dev = -1;
+  HIP_TRY(c, hipGetDevice(&dev));
+  if (dev != c->device) return fail(c, AWFL_E_ARG, "awfl_time_step_noalloc: the context's device is not current");
+  int dirs[3], nd = 0;
+  static const int fwd[3] = {0, 1, 2}, bwd[3] = {2, 1, 0};
+  for (int s = 0; s < 3; s++) {
+    int d = c->dimSwitch ? fwd[s] : bwd[s];
+    if (d == 1 && c->sim2d) continue;                // tendency is zero (S:572-575)
+    dirs[nd++] = d;
+  }
+  // One scratch state S.  Two sweeps: state -> S -> state.  Three: the x
+  // sweep runs in place (its work-group owns the whole line), so x, y, z is
+  // state -> state -> S -> state and z, y, x is state -> S -> state -> state.
+  double* S = c->noalloc;
+  for (int s = 0; s < nd; s++) {
+    const int d = dirs[s];
+    const double* src;
+    double* dst;
+    if (nd == 2) {
+      src = s == 0 ? d_state : S;
+      dst = s == 0 ? S : d_state;
+    } else if (d == 0) {
+      src = dst = d_state;
+    } else {
+      const bool first_after_state = (dirs[0] == 0) ? s == 1 : s == 0;
+      src = first_after_state ? d_state : S;
+      dst = first_after_state ? S : d_state;
+    }
+    SweepK K = sweep_constants(c, dt, d);
+    int rc = (d == 0 && src == dst)
+                 ? launch_sweep_with(c, src, dst, K, 0, v->sweep_x_inplace, nullptr, true)
+                 : launch_sweep_with(c, src, dst, K, d, v->sweep_x[0], d == 1 ? v->sweep_y[0] : v->sweep_z[0]);
+    if (rc != AWFL_OK) return rc;
+  }
+  c->dimSwitch = !c->dimSwitch;
+  return AWFL_OK;
 }
 
 int awfl_time_step(awfl_ctx* c, double* d_state, double dt) {

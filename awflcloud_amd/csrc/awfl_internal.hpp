@@ -121,6 +121,10 @@ struct awfl_ctx {
   unsigned tpos = 0;
   double* t_buf[2] = {nullptr, nullptr};
   int n_red = 0;
+  // this device's copy of the file-scope scratch of awfl_time_step_noalloc
+  // (null, with the reason in noalloc_err, if its address was not obtained)
+  double* noalloc = nullptr;
+  std::string noalloc_err;
   // sweep timing of awfl_time_step (awfl_sweep_timing): event pairs recorded
   // on the stream around each sweep launch, with the sweep's direction
   std::vector<hipEvent_t> t_ev;

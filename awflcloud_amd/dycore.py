@@ -13,6 +13,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libawfl_dycore.so")
+# per-device scratch of awfl_time_step_noalloc (AWFL_NOALLOC_MAX_BYTES)
+NOALLOC_MAX_BYTES = 64 * 1024 * 1024
 
 AWFL_BC_PERIODIC, AWFL_BC_WALL = 0, 1
 AWFL_FP64, AWFL_FP32_PERT = 0, 1
@@ -100,6 +102,7 @@ _SIGNATURES = {
     "awfl_compute_tendencies": ([_P, _P, _P, _D, C.c_int], C.c_int),
     "awfl_apply_tendencies": ([_P, _P, _P, _D, C.c_int], C.c_int),
     "awfl_time_step": ([_P, _P, _D], C.c_int),
+    "awfl_time_step_noalloc": ([_P, _P, _D], C.c_int),
     "awfl_sweep_timing": ([_P, C.c_int], C.c_int),
     "awfl_sweep_times": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
     "awfl_time_step_branches": ([_P, _P, _D, _P, C.c_int], C.c_int),
@@ -277,6 +280,13 @@ class Dycore:
         n = (C.c_int64 * 3)()
         self._check(self.L.awfl_sweep_times(self.h, ms, n))
         return list(ms), list(n)
+
+    def time_step_noalloc(self, state, dt):
+        """time_step without any allocation (awfl_time_step_noalloc): its
+        intermediate state lives in the library's per-device scratch of
+        NOALLOC_MAX_BYTES; larger states raise."""
+        self._chk_state(state)
+        self._check(self.L.awfl_time_step_noalloc(self.h, self._ptr(state), dt))
 
     def new_branch_record(self):
         """uint8 device buffer for the upwind branches of one step (three

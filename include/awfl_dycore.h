@@ -130,6 +130,21 @@ int awfl_time_step_branches(awfl_ctx* ctx, double* d_state, double dt, unsigned 
  * ms and the number of timed sweeps since the previous call. */
 int awfl_sweep_timing(awfl_ctx* ctx, int max_sweeps);
 int awfl_sweep_times(awfl_ctx* ctx, double ms[3], int64_t count[3]);
+/* awfl_time_step that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and the same result, bit for bit, as awfl_time_step; its one
+ * state-sized intermediate lives in a file-scope device array of
+ * AWFL_NOALLOC_MAX_BYTES that each device holds from the library's first use
+ * on it until the process ends (the x sweep runs in place, so one
+ * intermediate suffices).  States larger than that are rejected with
+ * AWFL_E_ARG; (ord, ngll) = (9, 5), whose kernels spill to scratch memory that
+ * the HIP runtime allocates on their first launch, with AWFL_E_UNSUPPORTED.
+ * All contexts on one device share that array: calls that
+ * overlap in time on one device (other streams, other threads) race; calls
+ * in sequence do not.  The context's device must be current. */
+#define AWFL_NOALLOC_MAX_BYTES (64ll * 1024 * 1024)
+int awfl_time_step_noalloc(awfl_ctx* ctx, double* d_state, double dt);
 /* One fused split sweep in direction dir (0 x, 1 y, 2 z): d_out = d_in + dt*tend.
  * d_in and d_out must not alias. */
 int awfl_sweep(awfl_ctx* ctx, const double* d_in, double* d_out, double dt, int dir);
