@@ -35,6 +35,7 @@
 #if AWFL_PARITY & AWFL_PAR_ORDER
 #include "awfl_cell_ref.hpp"
 #endif
+#include "awfl_sweep_line.hpp"
 
 // Cell body of the sweeps: the production numerics, or (reference-order
 // parity builds) the reference's operation order.
@@ -605,6 +606,15 @@ constexpr void (*inplace_x())(const double*, double*, Grid, SweepK) {
   else return k_sweep_x<O, N, false, 0, true>;
 }
 
+// the lane-split line sweeps (awfl_sweep_line.hpp), used for (9, 5), whose
+// per-cell kernels need more registers than a lane has
+using LineFn = void (*)(const double*, double*, Grid, SweepK);
+template <int O, int N, int D, bool T>
+constexpr LineFn line_fn() {
+  if constexpr (O == 9) return awfl_line::k_sweep_line<O, N, D, T>;
+  else return nullptr;
+}
+
 struct Variant {
   int ord, ngll;
   void (*sweep_x[2])(const double*, double*, Grid, SweepK);
@@ -612,15 +622,31 @@ struct Variant {
   void (*sweep_y[2])(const double*, double*, Grid, SweepK, int);
   void (*sweep_z[2])(const double*, double*, Grid, SweepK, int);
   void (*init)(double*, Grid, InitDev);
+  LineFn sweep_line[3][2];             // [dir][tend]; null: the per-cell kernels above
 };
+
+// per-cell sweep kernels, for the orders that do not use the line kernels
+template <int O, int N, bool T>
+constexpr void (*cell_x())(const double*, double*, Grid, SweepK) {
+  if constexpr (O == 9) return nullptr;
+  else return k_sweep_x<O, N, T>;
+}
+template <int O, int N, int D, bool T>
+constexpr void (*cell_col())(const double*, double*, Grid, SweepK, int) {
+  if constexpr (O == 9) return nullptr;
+  else return k_sweep_col<O, N, D, T>;
+}
 
 #define AWFL_VARIANT(O, N)                                                                        \
   Variant{O, N,                                                                                   \
-          {k_sweep_x<O, N, false>, k_sweep_x<O, N, true>},                                        \
+          {cell_x<O, N, false>(), cell_x<O, N, true>()},                                          \
           inplace_x<O, N>(),                                                                      \
-          {k_sweep_col<O, N, 1, false>, k_sweep_col<O, N, 1, true>},                              \
-          {k_sweep_col<O, N, 2, false>, k_sweep_col<O, N, 2, true>},                              \
-          k_init<O>}
+          {cell_col<O, N, 1, false>(), cell_col<O, N, 1, true>()},                                \
+          {cell_col<O, N, 2, false>(), cell_col<O, N, 2, true>()},                                \
+          k_init<O>,                                                                              \
+          {{line_fn<O, N, 0, false>(), line_fn<O, N, 0, true>()},                                 \
+           {line_fn<O, N, 1, false>(), line_fn<O, N, 1, true>()},                                 \
+           {line_fn<O, N, 2, false>(), line_fn<O, N, 2, true>()}}}
 
 #ifdef AWFL_LEAN
 // parity builds: the FP64 (5, 3) sweeps only, no FP32 form in the library
@@ -1006,9 +1032,24 @@ static int launch_sweep_with(awfl_ctx* c, const double* in, double* out, const S
   return AWFL_OK;
 }
 
+// one lane-split line sweep: a work-group of awfl_line::LS_THREADS per line
+static int launch_line(awfl_ctx* c, const double* in, double* out, const SweepK& K, int dir, LineFn k) {
+  const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
+  const unsigned lines = (unsigned)((int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz / n);
+  const size_t lds = sizeof(double) * awfl_line::line_lds_doubles(n, c->hs, c->cfg.ngll);
+  if (lds > 160 * 1024) return fail(c, AWFL_E_UNSUPPORTED, "line too long for the lane-split sweep's LDS");
+  if (lds > 64 * 1024)
+    HIP_TRY(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k<<<lines, awfl_line::LS_THREADS, lds, c->stream>>>(in, out, c->grid(), K);
+  LAUNCH_CHECK(c);
+  return AWFL_OK;
+}
+
 // launch one sweep; tend selects the tendency output (layout [5][nz][ny][nx])
 static int launch_sweep(awfl_ctx* c, const double* in, double* out, double dt, int dir, bool tend) {
   const Variant* v = static_cast<const Variant*>(c->var);
+  if (v->sweep_line[dir][0])
+    return launch_line(c, in, out, sweep_constants(c, dt, dir), dir, v->sweep_line[dir][tend ? 1 : 0]);
   return launch_sweep_with(c, in, out, sweep_constants(c, dt, dir), dir, v->sweep_x[tend ? 1 : 0],
                            dir == 1 ? v->sweep_y[tend ? 1 : 0] : v->sweep_z[tend ? 1 : 0]);
 }
