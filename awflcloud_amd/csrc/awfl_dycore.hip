@@ -597,12 +597,11 @@ __global__ void k_init(double* __restrict__ st, Grid G, InitDev S) {
 // ---------------------------------------------------------------------------
 // compiled (ord, ngll) variants
 // ---------------------------------------------------------------------------
-// the in-place x kernel of awfl_time_step_noalloc, built where the sweeps use
-// no scratch memory (the (9,5) kernels spill, and the runtime grows its
-// scratch pool on their first launch, an allocation the caller cannot avoid)
+// the in-place x kernel of awfl_time_step_noalloc: the per-cell x kernel, or
+// for (9, 5) the x line kernel
 template <int O, int N>
 constexpr void (*inplace_x())(const double*, double*, Grid, SweepK) {
-  if constexpr (O == 9) return nullptr;
+  if constexpr (O == 9) return awfl_line::k_sweep_line<O, N, 0, false, true>;
   else return k_sweep_x<O, N, false, 0, true>;
 }
 
@@ -1033,10 +1032,11 @@ static int launch_sweep_with(awfl_ctx* c, const double* in, double* out, const S
 }
 
 // one lane-split line sweep: a work-group of awfl_line::LS_THREADS per line
-static int launch_line(awfl_ctx* c, const double* in, double* out, const SweepK& K, int dir, LineFn k) {
+static int launch_line(awfl_ctx* c, const double* in, double* out, const SweepK& K, int dir, LineFn k,
+                       bool inplace = false) {
   const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
   const unsigned lines = (unsigned)((int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz / n);
-  const size_t lds = sizeof(double) * awfl_line::line_lds_doubles(n, c->hs, c->cfg.ngll);
+  const size_t lds = sizeof(double) * awfl_line::line_lds_doubles(n, c->hs, c->cfg.ngll, inplace);
   if (lds > 160 * 1024) return fail(c, AWFL_E_UNSUPPORTED, "line too long for the lane-split sweep's LDS");
   if (lds > 64 * 1024)
     HIP_TRY(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1157,9 +1157,6 @@ int awfl_time_step_noalloc(awfl_ctx* c, double* d_state, double dt) {
   const int64_t lds = (int64_t)sizeof(double) * (5 * (c->cfg.nx + 2 * c->hs) + 15 * c->cfg.nx + 5);
   if (lds > 160 * 1024) return fail(c, AWFL_E_UNSUPPORTED, "awfl_time_step_noalloc: nx too large for the in-place x line");
   const Variant* v = static_cast<const Variant*>(c->var);
-  if (!v->sweep_x_inplace)
-    return fail(c, AWFL_E_UNSUPPORTED, "awfl_time_step_noalloc: the (9, 5) sweeps spill to scratch memory, which the "
-                                       "HIP runtime allocates on their first launch; use awfl_time_step");
   if (!c->noalloc) return fail(c, AWFL_E_HIP, "awfl_time_step_noalloc: no scratch address: " + c->noalloc_err);
   int dev = -1;
   HIP_TRY(c, hipGetDevice(&dev));
@@ -1190,9 +1187,14 @@ int awfl_time_step_noalloc(awfl_ctx* c, double* d_state, double dt) {
       dst = first_after_state ? S : d_state;
     }
     SweepK K = sweep_constants(c, dt, d);
-    int rc = (d == 0 && src == dst)
-                 ? launch_sweep_with(c, src, dst, K, 0, v->sweep_x_inplace, nullptr, true)
-                 : launch_sweep_with(c, src, dst, K, d, v->sweep_x[0], d == 1 ? v->sweep_y[0] : v->sweep_z[0]);
+    int rc;
+    if (v->sweep_line[d][0])          // (9, 5): line kernels
+      rc = (d == 0 && src == dst) ? launch_line(c, src, dst, K, 0, v->sweep_x_inplace, true)
+                                  : launch_line(c, src, dst, K, d, v->sweep_line[d][0]);
+    else
+      rc = (d == 0 && src == dst)
+               ? launch_sweep_with(c, src, dst, K, 0, v->sweep_x_inplace, nullptr, true)
+               : launch_sweep_with(c, src, dst, K, d, v->sweep_x[0], d == 1 ? v->sweep_y[0] : v->sweep_z[0]);
     if (rc != AWFL_OK) return rc;
   }
   c->dimSwitch = !c->dimSwitch;

@@ -34,11 +34,9 @@
 #include "awfl_cell.hpp"
 #include "awfl_internal.hpp"
 
-// Waves per SIMD the line kernels are compiled for.  At (9, 5) they need
-// 176-185 VGPRs unbounded (two waves); bounded to three waves they take 168
-// and spill 48-80 bytes per lane, and run 15-17% faster (192x192x96: x 0.70
-// against 0.85 ms, y 0.73 / 0.90, z 0.90 / 1.03); bounded to four, 128 VGPRs
-// and 208-240 bytes of spill, 1.6x slower.
+// Waves per SIMD the line kernels are compiled for.  Bounded to three, the
+// (9, 5) kernels take 150 (x, y) and 158 (z) VGPRs and no scratch; bounded
+// to four they spill 208-240 bytes per lane and run 1.6x slower.
 #ifndef AWFL_LS_WAVES
 #define AWFL_LS_WAVES 3
 #endif
@@ -61,12 +59,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // LDS of one line of n cells: staged line (reused for the face fluxes), GLL
 // values / flux exchange of a chunk, the source value of each chunk cell's
-// last GLL point, edge states, z source averages
-inline size_t line_lds_doubles(int n, int hs, int ngll) {
-  return (size_t)5 * (n + 2 * hs) + (size_t)5 * ngll * LS_CHUNK + LS_CHUNK + (size_t)10 * n + n;
+// last GLL point, edge states, z source averages (and, in place, the face
+// fluxes apart from the line)
+inline size_t line_lds_doubles(int n, int hs, int ngll, bool inplace = false) {
+  return (size_t)5 * (n + 2 * hs) + (size_t)5 * ngll * LS_CHUNK + LS_CHUNK + (size_t)10 * n + n +
+         (inplace ? (size_t)5 * (n + 1) : 0);
 }
 
-template <int ORD, int NGLL, int DIR, bool TEND>
+// INPLACE (in == out allowed): every global read of the line happens while
+// staging it, before the first barrier; the face fluxes then get LDS of their
+// own and the update takes each cell's old value from the staged line.
+template <int ORD, int NGLL, int DIR, bool TEND, bool INPLACE = false>
 __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL_LS_WAVES))) void k_sweep_line(const double* __restrict__ in, double* __restrict__ out,
                                                            Grid G, SweepK K) {
   constexpr int hs = (ORD - 1) / 2;
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
   double* s_sv = s_x + 5 * NGLL * LS_CHUNK;            // [CHUNK]
   double* s_e = s_sv + LS_CHUNK;                       // [10][n]: eL | eR
   double* s_src = s_e + 10 * n;                        // [n]: z source averages
-  double* s_F = sm;
+  double* s_F = INPLACE ? s_src + n : sm;               // [5][n+1]
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
 
   // 1. stage the line
@@ -133,7 +136,6 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
   // derivative-matrix column of this lane's point: d(ii) = sum_s D[s][ii] F(s)
   double Dc[NGLL];
   sfor<NGLL>([&](auto s) { Dc[s] = Deriv<NGLL>::m[s][ii]; });
-  const double wt = Gll<NGLL>::wts[ii];
 
   for (int c0 = 0; c0 < n; c0 += LS_CHUNK) {
     const int cnt = min(LS_CHUNK, n - c0);
@@ -153,12 +155,22 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
     const int cl = cl_t;
     const int c = c0 + cl;
     const bool act = lane_ok && cl < cnt;
+    // LDS slot of this lane's cell: an idle lane reads the last cell's
+    // values and stores nothing
+    const int clr = min(cl, LS_CHUNK - 1);
     const int kc = DIR == 2 ? min(c, n - 1) : kl;       // level of the cell's background
-    double q[5][NA], f[4][NA], g[NA], src[NA];
+    double q[5][NA], f[4][NA], g[NA];
+    double src0 = 0.0;                                  // z: the source at level 0 (Q2 broadcast)
+    // z gravity source at level kt: the level-0 value, then -rho_kt g (S:2055),
+    // formed where it is used rather than kept
+    auto src = [&](auto kt) -> double {
+      if constexpr (kt == 0) return src0;
+      else return -q[idR][kt] * GRAV;
+    };
     double rr = 1.0, rrt = 1.0;
     {
       double q0[5];
-      sfor<5>([&](auto l) { q0[l] = act ? s_x[(l * NGLL + ii) * LS_CHUNK + cl] : 0.0; });
+      sfor<5>([&](auto l) { q0[l] = s_x[(l * NGLL + ii) * LS_CHUNK + clr]; });
       double brho, brt;
       PowRef pr;
       if constexpr (DIR == 2) {
@@ -196,11 +208,10 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
         const double sv = -(q[idR][0] - brho) * GRAV;
         if (act && ii == NGLL - 1) s_sv[cl] = sv;
         wave_lds_sync();
-        const double svl = act ? s_sv[cl] : 0.0;
-        sfor<NA>([&](auto kt) { src[kt] = svl; });
+        const double svl = s_sv[clr];
+        src0 = svl;
       }
     }
-    const double pGii = DIR == 2 ? G.hyPressureGLL[kc * NGLL + ii] : 0.0;
     sfor<NA - 1>([&](auto kt) {
       constexpr double rkt1 = 1.0 / (kt + 1.0);
       // this point's five flux values at level kt into the exchange rows
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
         constexpr int l = m + 1;
         if constexpr (l == NM) {
           if constexpr (kt == 0) {
-            if constexpr (DIR == 2) Fl[l] = f[m][kt] + C0 * g[kt] - pGii;
+            if constexpr (DIR == 2) Fl[l] = f[m][kt] + C0 * g[kt] - G.hyPressureGLL[kc * NGLL + ii];
             else Fl[l] = f[m][kt] + C0 * g[kt];
           } else {
             Fl[l] = f[m][kt] + (0.5 * C0) * g[kt];
@@ -225,18 +236,17 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
       double d[5];
       sfor<5>([&](auto l) {
         double t = -0.0;
-        sfor<NGLL>([&](auto s) { t += Dc[s] * (act ? s_x[(l * NGLL + s) * LS_CHUNK + cl] : 0.0); });
+        sfor<NGLL>([&](auto s) { t += Dc[s] * s_x[(l * NGLL + s) * LS_CHUNK + clr]; });
         d[l] = t;
       });
       const double ndt = K.ndt[kt];
       sfor<5>([&](auto l) {
-        if constexpr (DIR == 2 && l == idW) q[l][kt + 1] = d[l] * ndt + src[kt] * rkt1;
+        if constexpr (DIR == 2 && l == idW) q[l][kt + 1] = d[l] * ndt + src(kt) * rkt1;
         else q[l][kt + 1] = d[l] * ndt;
       });
       if constexpr (DIR == 2) {                         // S:2051-2055
         if (!per && c == n - 1 && ii == NGLL - 1) q[idW][kt + 1] = 0.0;
         if (!per && c == 0 && ii == 0) q[idW][kt + 1] = 0.0;
-        src[kt + 1] = -q[idR][kt + 1] * GRAV;
       }
       if constexpr (kt + 2 < NA) {                      // products at level kt+1
         sfor<4>([&](auto m) {
@@ -269,10 +279,10 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
       });
     }
     if constexpr (DIR == 2) {                           // S:1462, 1489-1492
-      double s0 = src[0];
-      sfor<NA - 1>([&](auto km) { s0 += src[km + 1] * K.tav[km + 1]; });
+      double s0 = src(std::integral_constant<int, 0>{});
+      sfor<NA - 1>([&](auto km) { s0 += src(std::integral_constant<int, km + 1>{}) * K.tav[km + 1]; });
       wave_lds_sync();
-      if (act) s_x[ii * LS_CHUNK + cl] = s0 * wt;
+      if (act) s_x[ii * LS_CHUNK + cl] = s0 * Gll<NGLL>::wts[ii];
       wave_lds_sync();
       if (act && ii == 0) {
         double sa = 0.0;
@@ -337,7 +347,8 @@ __global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(AWFL
         out[ti] = td;
       } else {
         const int64_t o = l * G.sl + base + (int64_t)c * stride;
-        out[o] = in[o] + K.dt * td;
+        if constexpr (INPLACE) out[o] = s_st[l * W + hs + c] + K.dt * td;
+        else out[o] = in[o] + K.dt * td;
       }
     });
   }

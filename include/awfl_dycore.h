@@ -138,9 +138,7 @@ int awfl_sweep_times(awfl_ctx* ctx, double ms[3], int64_t count[3]);
  * AWFL_NOALLOC_MAX_BYTES that each device holds from the library's first use
  * on it until the process ends (the x sweep runs in place, so one
  * intermediate suffices).  States larger than that are rejected with
- * AWFL_E_ARG; (ord, ngll) = (9, 5), whose kernels spill to scratch memory that
- * the HIP runtime allocates on their first launch, with AWFL_E_UNSUPPORTED.
- * All contexts on one device share that array: calls that
+ * AWFL_E_ARG.  All contexts on one device share that array: calls that
  * overlap in time on one device (other streams, other threads) race; calls
  * in sequence do not.  The context's device must be current. */
 #define AWFL_NOALLOC_MAX_BYTES (64ll * 1024 * 1024)
