@@ -347,10 +347,37 @@ __global__ __launch_bounds__(64) void k_weno_adj(const double* __restrict__ st, 
 // state line [5][n+2hs], a region [5 NGLL][n] for the WENO values of each
 // cell and then, in place, their adjoints q0b, and a region [10][n] for the
 // edge states, replaced in place by the edge-state adjoints (every slot is
-// read and written by the one face thread that owns it) and then reused,
-// two variables at a time, for the stencil adjoints; then the line's
-// constants and adj_out along the line [5][n], staged once.
+// read and written by the one face thread that owns it); the two regions
+// then hold the stencil adjoints of all five variables (adj_row); then the
+// line's constants and adj_out along the line [5][n], staged once.
 // ---------------------------------------------------------------------------
+// WENO reverses per thread side by side in the line kernel
+template <int ORD>
+constexpr int adj_pv() {
+  return 2 * ORD <= 10 ? 2 : 1;
+}
+// row of the stencil adjoint of variable l, slot s, in the line kernel's row
+// region [s_q | s_e] of 5 NGLL + 10 rows: variable after variable from the
+// first edge row on, wrapping to row 0
+template <int ORD, int NGLL>
+__host__ __device__ constexpr int adj_row(int l, int s) {
+  return (5 * NGLL + l * ORD + s) % (5 * NGLL + 10);
+}
+// every stencil adjoint has a row of its own, and none lands on the q0b rows
+// (l NGLL .. l NGLL + NGLL - 1) of a variable whose WENO reverse runs in a
+// later group of PV
+template <int ORD, int NGLL, int PV>
+constexpr bool adj_rows_ok() {
+  if (5 * ORD > 5 * NGLL + 10) return false;
+  for (int l = 0; l < 5; ++l) {
+    const int next = (l / PV + 1) * PV < 5 ? (l / PV + 1) * PV : 5;
+    for (int s = 0; s < ORD; ++s) {
+      const int r = adj_row<ORD, NGLL>(l, s);
+      if (r >= next * NGLL && r < 5 * NGLL) return false;
+    }
+  }
+  return true;
+}
 template <int ORD, int NGLL>
 constexpr int adj_line_rows() {
   return 5 * NGLL + 10;
@@ -418,24 +445,47 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       return fr;
     }
   };
-  if constexpr (DIR != 2) {
-    if (threadIdx.x == 0) {
-      const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, kl);
-      const FaceRef fr = face_ref<DIR>(G, kl, 0);
-      s_k[0] = bg.r;
-      s_k[1] = bg.rt;
-      s_k[2] = bg.pt.h, s_k[3] = bg.pt.l, s_k[4] = bg.pt.rb, s_k[5] = bg.pt.b;
-      s_k[6] = fr.rt.h, s_k[7] = fr.rt.l, s_k[8] = fr.rt.rb, s_k[9] = fr.rt.b;
+  // 1. stage the line (boundary rule at load), adj_out along it and an x or
+  // y line's constants.  Every global load of a thread is issued before its
+  // first LDS store, so the stage waits for memory once (each thread holds
+  // at most one halo cell, and one cell of the line when blockDim >= n).
+  {
+    const int t = threadIdx.x;
+    const bool halo = t < 2 * hs;
+    const int hc = t < hs ? t : n + t;  // padded slot of halo cell t
+    double hv[5];
+    if (halo) {
+      bool zn;
+      const int src = map_axis(hc - hs, n, per, zn);
+      sfor<5>([&](auto l) { hv[l] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
+    }
+    double kv[LINE_K];
+    if constexpr (DIR != 2) {
+      if (t == 0) {
+        const CellBg<NGLL> bg = cell_bg<NGLL, DIR>(G, kl);
+        const FaceRef fr = face_ref<DIR>(G, kl, 0);
+        kv[0] = bg.r;
+        kv[1] = bg.rt;
+        kv[2] = bg.pt.h, kv[3] = bg.pt.l, kv[4] = bg.pt.rb, kv[5] = bg.pt.b;
+        kv[6] = fr.rt.h, kv[7] = fr.rt.l, kv[8] = fr.rt.rb, kv[9] = fr.rt.b;
+      }
+    }
+    for (int c = t; c < n; c += blockDim.x) {
+      double v[5], a[5];
+      sfor<5>([&](auto l) {
+        v[l] = in[l * G.sl + base + c * stride];
+        a[l] = adj_out[l * G.sl + base + c * stride];
+      });
+      sfor<5>([&](auto l) {
+        s_st[l * W + hs + c] = v[l];
+        s_ao[l * n + c] = a[l];
+      });
+    }
+    if (halo) sfor<5>([&](auto l) { s_st[l * W + hc] = hv[l]; });
+    if constexpr (DIR != 2) {
+      if (t == 0) sfor<LINE_K>([&](auto k) { s_k[k] = kv[k]; });
     }
   }
-  // 1. stage the line (boundary rule at load) and adj_out along it
-  for (int c = threadIdx.x; c < W; c += blockDim.x) {
-    bool zn;
-    const int src = map_axis(c - hs, n, per, zn);
-    sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
-  }
-  for (int c = threadIdx.x; c < n; c += blockDim.x)
-    sfor<5>([&](auto l) { s_ao[l * n + c] = adj_out[l * G.sl + base + c * stride]; });
   __syncthreads();
   // 2. cell forward: WENO values (independent chains the scheduler
   // interleaves: two variables at a time in x, all five in y and z, as
@@ -514,14 +564,17 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     adj::cell_dt_adj<NGLL, DIR>(q0, bg_of(c), K, eLb, eRb, srcb, q0b);
     sfor<5>([&](auto l) { sfor<NGLL>([&](auto ii) { s_q[(l * NGLL + ii) * n + c] = q0b[l][ii]; }); });
   }
-  // 4b. WENO reverse into the edge region (each cell's own column, whose
-  // edge adjoints step 4a has consumed), PV variables at a time side by side
-  // (two independent reverses per thread give the scheduler two chains to
-  // interleave), then the gather of the stencil adjoints that point at each
-  // cell
-  constexpr int PV = 2 * ORD <= 10 ? 2 : 1;
-  // variables l0 .. l0+NV-1: stencil adjoints of variable l0+v in rows v*ORD+s
-  auto weno_rev = [&](int l0, auto nv) {
+  // 4b. WENO reverse in each cell's own column, PV variables at a time side
+  // by side (independent chains the scheduler interleaves).  The stencil
+  // adjoints of every variable go to the row region [s_q | s_e] (5 NGLL + 10
+  // rows; adj_row), in rows that hold no q0b of a later group, so one
+  // barrier and one gather of the stencil adjoints that point at each cell
+  // finish the sweep.
+  static_assert(adj_rows_ok<ORD, NGLL, adj_pv<ORD>()>(), "stencil adjoints fit the row region");
+  constexpr int PV = adj_pv<ORD>();
+  double* s_r = s_q;
+  auto weno_rev = [&](auto l0c, auto nv) {
+    constexpr int l0 = decltype(l0c)::value;
     constexpr int NV = decltype(nv)::value;
     bool all = true;
     sfor<NV>([&](auto v) { all = all && dow(l0 + v); });
@@ -539,9 +592,15 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       } else {
         sfor<NV>([&](auto v) { adj::reconstruct_adj<ORD, NGLL>(u[v], dow(l0 + v), gb[v], ub[v]); });
       }
-      sfor<NV>([&](auto v) { sfor<ORD>([&](auto s) { s_e[(v * ORD + s) * n + c] = ub[v][s]; }); });
+      sfor<NV>([&](auto v) {
+        sfor<ORD>([&](auto s) { s_r[adj_row<ORD, NGLL>(l0 + v, s) * n + c] = ub[v][s]; });
+      });
     }
   };
+  sfor<5 / PV>([&](auto g) { weno_rev(std::integral_constant<int, g * PV>{}, std::integral_constant<int, PV>{}); });
+  if constexpr (5 % PV != 0)
+    weno_rev(std::integral_constant<int, 5 - 5 % PV>{}, std::integral_constant<int, 5 % PV>{});
+  __syncthreads();
   // src = c + hs - s is the cell whose stencil slot s is c: a periodic line
   // wraps it (by one period when n >= hs, else by the remainder), a wall
   // drops it when it falls outside the line.  At a wall the stencil slots
@@ -553,11 +612,10 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
       sfor<NV>([&](auto v) {
         const int l = l0 + v;
-        const double* se = s_e + v * ORD * n;
         double acc = s_ao[l * n + c];
         sfor<ORD>([&](auto s) {
           if constexpr (s == hs) {
-            acc += se[s * n + c];
+            acc += s_r[adj_row<ORD, NGLL>(l, s) * n + c];
           } else {
             int src = c + hs - s;
             if constexpr (decltype(small)::value) {
@@ -568,7 +626,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
               src -= src >= n ? wrap : 0;
             }
             const bool ok = src >= 0 && src < n;
-            const double val = se[s * n + (ok ? src : c)];
+            const double val = s_r[adj_row<ORD, NGLL>(l, s) * n + (ok ? src : c)];
             acc += ok ? val : 0.0;
           }
         });
@@ -577,7 +635,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
             sfor<hs>([&](auto cc) {
               sfor<ORD>([&](auto s) {
                 if constexpr (cc - hs + s < 0) {
-                  if (cc < n) acc += se[s * n + cc];
+                  if (cc < n) acc += s_r[adj_row<ORD, NGLL>(l, s) * n + cc];
                 }
               });
             });
@@ -586,7 +644,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
               const int cc = n - 1 - m;
               sfor<ORD>([&](auto s) {
                 if constexpr (s >= hs + 1 + m) {
-                  if (cc >= 0) acc += se[s * n + cc];
+                  if (cc >= 0) acc += s_r[adj_row<ORD, NGLL>(l, s) * n + cc];
                 }
               });
             });
@@ -595,16 +653,8 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       });
     }
   };
-  auto step = [&](int l0, auto nv) {
-    weno_rev(l0, nv);
-    __syncthreads();
-    if (per && n < hs) gather(l0, nv, std::true_type{});
-    else gather(l0, nv, std::false_type{});
-    __syncthreads();
-  };
-#pragma unroll 1
-  for (int l0 = 0; l0 + PV <= 5; l0 += PV) step(l0, std::integral_constant<int, PV>{});
-  if constexpr (5 % PV != 0) step(5 - 5 % PV, std::integral_constant<int, 5 % PV>{});
+  if (per && n < hs) gather(0, std::integral_constant<int, 5>{}, std::true_type{});
+  else gather(0, std::integral_constant<int, 5>{}, std::false_type{});
   // the halo cells of an x line (y and z halos: k_zero_halo)
   if constexpr (DIR == 0) {
     for (int c = threadIdx.x; c < 2 * hs; c += blockDim.x) {
@@ -968,6 +1018,7 @@ int record_steps(awfl_ctx* c, const double* start, int n0, int n1, int ds0, doub
 }  // namespace
 
 extern "C" {
+
 
 int awfl_sweep_tangent(awfl_ctx* c, const double* d_in, const double* d_din, double* d_out, double* d_dout,
                        double dt, int dir) {
