@@ -80,10 +80,20 @@ __device__ __forceinline__ int wrap_axis(int a, int n, bool per, bool& zn) {
 //   again from the state edges, which stay in place to the end.
 // Divisions by the density and by the edge densities are multiplications by
 // one reciprocal (rcp, ~1 ulp) shared by all tracers.
+//
+// With one cell per thread (every line up to 256 cells) a thread issues all
+// loads of its cell and of one halo cell before its first LDS store (one
+// memory round trip to stage the line), and the state and tracer values of
+// its cell that the limiter and the update read again are loaded when the
+// faces start, in flight behind them.  Register arrays per tracer have
+// AWFL_MAX_TRACERS entries, each used under `t < ntr`.  (Compiling the
+// kernel for exactly three tracers was measured slower: the unrolled
+// tracer loops took another 16 VGPRs, one wave per SIMD fewer.)
 template <int ORD, int NGLL, int DIR, bool TEND>
 __device__ __forceinline__ void sweep_tr(const double* __restrict__ in, const double* __restrict__ tr_in,
                                          double* __restrict__ out, double* __restrict__ tr_out, const Grid& G,
                                          const SweepK& K, int ntr, unsigned pos) {
+  constexpr int MT = AWFL_MAX_TRACERS;
   constexpr int hs = (ORD - 1) / 2;
   constexpr int NM = 1 + DIR;
   constexpr int NA = NGLL;
@@ -122,22 +132,67 @@ __device__ __forceinline__ void sweep_tr(const double* __restrict__ in, const do
   const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
   auto level = [&](int c) { return DIR == 2 ? c : kl; };
 
-  // 1. the state line (normal momentum zero in wall ghosts), then the
-  // tracers' mixing ratios (ghosts copy the end cell at a wall)
-  for (int c = threadIdx.x; c < W; c += blockDim.x) {
-    bool zn;
-    const int src = wrap_axis(c - hs, n, per, zn);
-    sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
-  }
-  __syncthreads();
-  // full density of every cell (S:624-626)
-  for (int c = threadIdx.x; c < n; c += blockDim.x) s_dens[c] = s_st[idR * W + c + hs] + G.hyDensCells[hs + level(c)];
-  __syncthreads();
-  for (int c = threadIdx.x; c < W; c += blockDim.x) {
-    bool zn;
-    const int src = wrap_axis(c - hs, n, per, zn);
-    const double rr = rcp(s_dens[src]);
-    for (int t = 0; t < ntr; t++) s_tq[t * W + c] = tr_in[t * G.sl + base + src * stride] * rr;
+  const int tid = threadIdx.x;
+  const bool one = n <= (int)blockDim.x;
+
+  // 1. the state line (normal momentum zero in wall ghosts), the full
+  // density of every cell (S:624-626) and the tracers' mixing ratios (ghosts
+  // copy the end cell at a wall)
+  if (one) {
+    // ts: the thread index behind an empty asm, so that the compiler shares
+    // no address arithmetic between the stage and the cells (it would keep it
+    // in registers through them)
+    int ts = tid;
+    asm volatile("" : "+v"(ts));
+    const int hc = ts < hs ? ts : n + ts;  // padded slot of halo cell ts (ts < 2hs)
+    double v[5], q[MT], hv[5], hq[MT], bgc = 0.0, bgh = 0.0;
+    bool hzn = false;
+    if (ts < n) {
+      sfor<5>([&](auto l) { v[l] = in[l * G.sl + base + ts * stride]; });
+      sfor<MT>([&](auto t) {
+        if (t < ntr) q[t] = tr_in[t * G.sl + base + ts * stride];
+      });
+      bgc = G.hyDensCells[hs + level(ts)];
+    }
+    if (ts < 2 * hs) {
+      const int src = wrap_axis(hc - hs, n, per, hzn);
+      sfor<5>([&](auto l) { hv[l] = in[l * G.sl + base + src * stride]; });
+      sfor<MT>([&](auto t) {
+        if (t < ntr) hq[t] = tr_in[t * G.sl + base + src * stride];
+      });
+      bgh = G.hyDensCells[hs + level(src)];
+    }
+    if (ts < n) {
+      const double dens = v[idR] + bgc;
+      s_dens[ts] = dens;
+      sfor<5>([&](auto l) { s_st[l * W + hs + ts] = v[l]; });
+      const double rr = rcp(dens);
+      sfor<MT>([&](auto t) {
+        if (t < ntr) s_tq[t * W + hs + ts] = q[t] * rr;
+      });
+    }
+    if (ts < 2 * hs) {
+      sfor<5>([&](auto l) { s_st[l * W + hc] = (hzn && l == NM) ? 0.0 : hv[l]; });
+      const double rr = rcp(hv[idR] + bgh);
+      sfor<MT>([&](auto t) {
+        if (t < ntr) s_tq[t * W + hc] = hq[t] * rr;
+      });
+    }
+  } else {
+    for (int c = tid; c < W; c += blockDim.x) {
+      bool zn;
+      const int src = wrap_axis(c - hs, n, per, zn);
+      sfor<5>([&](auto l) { s_st[l * W + c] = (zn && l == NM) ? 0.0 : in[l * G.sl + base + src * stride]; });
+    }
+    __syncthreads();
+    for (int c = tid; c < n; c += blockDim.x) s_dens[c] = s_st[idR * W + c + hs] + G.hyDensCells[hs + level(c)];
+    __syncthreads();
+    for (int c = tid; c < W; c += blockDim.x) {
+      bool zn;
+      const int src = wrap_axis(c - hs, n, per, zn);
+      const double rr = rcp(s_dens[src]);
+      for (int t = 0; t < ntr; t++) s_tq[t * W + c] = tr_in[t * G.sl + base + src * stride] * rr;
+    }
   }
   __syncthreads();
 
@@ -230,6 +285,18 @@ __device__ __forceinline__ void sweep_tr(const double* __restrict__ in, const do
   // subtracts (S:1628-1629).
   const bool wrapF = per && DIR != 2;
   const int nfaces = wrapF ? n : n + 1;
+  // this thread's cell as it came in, for the limiter and the update; the
+  // empty asm keeps these loads after the barrier (issued earlier, their
+  // registers would be live through the cells)
+  double kin[5], ktr[MT];
+  if (one && tid < n) {
+    int64_t o = base + tid * stride;
+    asm volatile("" : "+v"(o));
+    sfor<5>([&](auto l) { kin[l] = in[l * G.sl + o]; });
+    sfor<MT>([&](auto t) {
+      if (t < ntr) ktr[t] = tr_in[t * G.sl + o];
+    });
+  }
   for (int f = threadIdx.x; f < nfaces; f += blockDim.x) {
     const bool wl = !per && f == 0, wr = !per && f == n;
     const int a = wl ? 0 : (f == 0 ? n - 1 : f - 1);
@@ -271,15 +338,17 @@ __device__ __forceinline__ void sweep_tr(const double* __restrict__ in, const do
     double* s_fac = s_te;              // [ntr][n]
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
       const int cp = (wrapF && c + 1 == n) ? 0 : c + 1;
-      for (int t = 0; t < ntr; t++) {
-        if (!((pos >> t) & 1u)) continue;
-        const double f1 = fmin(s_TF[t * NF + c], 0.0);
-        const double f2 = fmax(s_TF[t * NF + cp], 0.0);
-        const double fluxOut = K.dt * (f2 - f1) * K.rd;
-        // the cell's mixing ratio times its density, as staged in step 1
-        const double q = tr_in[t * G.sl + base + c * stride] * rcp(s_dens[c]);
-        s_fac[t * n + c] = fmin(1.0, q * s_dens[c] * rcp(fluxOut + TRACER_EPS));
-      }
+      sfor<MT>([&](auto t) {
+        if (t < ntr && ((pos >> t) & 1u)) {
+          const double f1 = fmin(s_TF[t * NF + c], 0.0);
+          const double f2 = fmax(s_TF[t * NF + cp], 0.0);
+          const double fluxOut = K.dt * (f2 - f1) * K.rd;
+          // the cell's mixing ratio times its density, as staged in step 1
+          const double m = one ? ktr[t] : tr_in[t * G.sl + base + c * stride];
+          const double q = m * rcp(s_dens[c]);
+          s_fac[t * n + c] = fmin(1.0, q * s_dens[c] * rcp(fluxOut + TRACER_EPS));
+        }
+      });
     }
     __syncthreads();
     for (int f = threadIdx.x; f < nfaces; f += blockDim.x) {
@@ -316,13 +385,15 @@ __device__ __forceinline__ void sweep_tr(const double* __restrict__ in, const do
       if (DIR == 2 && l == idW) td = s_src[c] + td;
       if (G.sim2d && l == idV) td = 0.0;
       if constexpr (TEND) out[l * nc + ot] = td;
-      else out[l * G.sl + o] = in[l * G.sl + o] + K.dt * td;
+      else out[l * G.sl + o] = (one ? kin[l] : in[l * G.sl + o]) + K.dt * td;
     });
-    for (int t = 0; t < ntr; t++) {
-      const double td = -(s_TF[t * NF + cp] - s_TF[t * NF + c]) * K.rd;
-      if constexpr (TEND) tr_out[t * nc + ot] = td;
-      else tr_out[t * G.sl + o] = tr_in[t * G.sl + o] + K.dt * td;
-    }
+    sfor<MT>([&](auto t) {
+      if (t < ntr) {
+        const double td = -(s_TF[t * NF + cp] - s_TF[t * NF + c]) * K.rd;
+        if constexpr (TEND) tr_out[t * nc + ot] = td;
+        else tr_out[t * G.sl + o] = (one ? ktr[t] : tr_in[t * G.sl + o]) + K.dt * td;
+      }
+    });
   }
 }
 
