@@ -393,6 +393,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   extern __shared__ double sm[];
   const int n = axis_len<DIR>(G);
   const int W = n + 2 * hs;
+  const int bd = line_threads(n);     // blockDim.x of the launch
   // line -> its first interior cell, the stride along DIR and the level k of
   // an x or y line
   const unsigned line = tile_index();
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   // 1. stage the line (boundary rule at load), adj_out along it and an x or
   // y line's constants.  Every global load of a thread is issued before its
   // first LDS store, so the stage waits for memory once (each thread holds
-  // at most one halo cell, and one cell of the line when blockDim >= n).
+  // at most one halo cell, and one cell of the line when n <= bd).
   {
     const int t = threadIdx.x;
     const bool halo = t < 2 * hs;
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
         kv[6] = fr.rt.h, kv[7] = fr.rt.l, kv[8] = fr.rt.rb, kv[9] = fr.rt.b;
       }
     }
-    for (int c = t; c < n; c += blockDim.x) {
+    for (int c = t; c < n; c += bd) {
       double v[5], a[5];
       sfor<5>([&](auto l) {
         v[l] = in[l * G.sl + base + c * stride];
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     else sfor<NV>([&](auto v) { reconstruct<ORD, NGLL>(w[v], dow(l0 + v), g[v]); });
     sfor<NV>([&](auto v) { sfor<NGLL>([&](auto ii) { s_q[((l0 + v) * NGLL + ii) * n + c] = g[v][ii]; }); });
   };
-  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+  for (int c = threadIdx.x; c < n; c += bd) {
     if constexpr (DIR == 0) {
 #pragma unroll 1
       for (int l0 = 0; l0 < 4; l0 += 2) recon(c, l0, std::integral_constant<int, 2>{});
@@ -522,7 +523,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   // 3. faces: flux adjoint -> edge-state adjoints, over the edge states
   const double cdr = K.dt * K.rd;
   const int nfaces = per ? n : n + 1;
-  for (int f = threadIdx.x; f < nfaces; f += blockDim.x) {
+  for (int f = threadIdx.x; f < nfaces; f += bd) {
     const bool wl = !per && f == 0, wr = !per && f == n;
     const int a = wl ? 0 : (f == 0 ? n - 1 : f - 1);      // cell on the low side
     const int b = wr ? n - 1 : (f == n ? 0 : f);          // cell on the high side
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   __syncthreads();
   // 4a. transform reverse: q0b over q0 (each cell's own column); in z the
   // source enters rho*w as out += dt * src
-  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+  for (int c = threadIdx.x; c < n; c += bd) {
     double q0[5][NGLL], q0b[5][NGLL], eLb[5], eRb[5];
     sfor<5>([&](auto l) {
       sfor<NGLL>([&](auto ii) { q0[l][ii] = s_q[(l * NGLL + ii) * n + c]; });
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     constexpr int NV = decltype(nv)::value;
     bool all = true;
     sfor<NV>([&](auto v) { all = all && dow(l0 + v); });
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    for (int c = threadIdx.x; c < n; c += bd) {
       double u[NV][ORD], ub[NV][ORD], gb[NV][NGLL];
       sfor<NV>([&](auto v) {
         sfor<ORD>([&](auto s) {
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   const int wrap = per ? n : 0;
   auto gather = [&](int l0, auto nv, auto small) {
     constexpr int NV = decltype(nv)::value;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    for (int c = threadIdx.x; c < n; c += bd) {
       sfor<NV>([&](auto v) {
         const int l = l0 + v;
         double acc = s_ao[l * n + c];
@@ -657,7 +658,7 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   else gather(0, std::integral_constant<int, 5>{}, std::false_type{});
   // the halo cells of an x line (y and z halos: k_zero_halo)
   if constexpr (DIR == 0) {
-    for (int c = threadIdx.x; c < 2 * hs; c += blockDim.x) {
+    for (int c = threadIdx.x; c < 2 * hs; c += bd) {
       const int64_t o = c < hs ? base - hs + c : base + n + (c - hs);
       sfor<5>([&](auto l) { adj_in[l * G.sl + o] = 0.0; });
     }
@@ -913,7 +914,7 @@ int sweep_adjoint(awfl_ctx* c, const double* in, const double* adj_out, double* 
     const unsigned lines = (unsigned)(nc / n);
     if (lds > 64 * 1024)
       HIP_TRY(c, hipFuncSetAttribute((const void*)v->line[dir], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int threads = std::min(256, (n + 63) / 64 * 64);
+    const int threads = line_threads(n);
     v->line[dir]<<<lines, threads, lds, c->stream>>>(in, adj_out, adj_in, G, K);
     LAUNCH_CHECK(c);
     if (zero_halo) {

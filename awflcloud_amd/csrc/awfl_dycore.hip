@@ -1009,7 +1009,7 @@ static int launch_sweep_with(awfl_ctx* c, const double* in, double* out, const S
   Grid G = c->grid();
   if (dir == 0) {
     int nx = c->cfg.nx;
-    int threads = std::min(256, (nx + 63) / 64 * 64);
+    int threads = line_threads(nx);
     size_t lds = sizeof(double) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx + (x_flux_lds ? 5 * (nx + 1) : 0));
     if (lds > 64 * 1024)
       HIP_TRY(c, hipFuncSetAttribute((const void*)kx, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -412,7 +412,7 @@ int launch_sweep_f32(awfl_ctx* c, const float* s, const float* cs, float* so, fl
   const VariantF* v = find_variant_f(c->cfg.ord, c->cfg.ngll);
   if (dir == 0) {
     int nx = c->cfg.nx;
-    int threads = std::min(256, (nx + 63) / 64 * 64);
+    int threads = line_threads(nx);
     size_t lds = sizeof(float) * (size_t)(5 * (nx + 2 * c->hs) + 10 * nx);
     if (lds > 64 * 1024)
       HIP_TRY(c, hipFuncSetAttribute((const void*)v->sweep_x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

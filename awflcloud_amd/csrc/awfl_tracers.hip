@@ -508,7 +508,7 @@ int launch_sweep_tr(awfl_ctx* c, const double* in, const double* tr_in, double* 
     HIP_TRY(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   const unsigned lines = (unsigned)(nc / n);
-  const int threads = std::min(256, (n + 63) / 64 * 64);
+  const int threads = line_threads(n);
   k<<<lines, threads, lds, c->stream>>>(in, tr_in, out, tr_out, c->grid(), sweep_constants(c, dt, dir), ntr,
                                         c->tpos);
   LAUNCH_CHECK(c);
