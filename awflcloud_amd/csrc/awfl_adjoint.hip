@@ -966,9 +966,9 @@ int ensure_states(awfl_ctx* c, int64_t nstates) {
   return AWFL_OK;
 }
 
-// reverse of the steps [n0, n1) whose split inputs are in c->a_states;
-// adj is updated in place (ping-pong through a_tmp)
-int reverse_steps(awfl_ctx* c, int n0, int n1, int ds0, double dt, double* adj) {
+// reverse of the steps [n0, n1) whose split inputs are in `states`; adj is
+// updated in place (ping-pong through a_tmp)
+int reverse_steps(awfl_ctx* c, const double* states, int n0, int n1, int ds0, double dt, double* adj) {
   const int64_t el = c->state_elems();
   double* cur = adj;
   double* oth = c->a_tmp;
@@ -977,7 +977,7 @@ int reverse_steps(awfl_ctx* c, int n0, int n1, int ds0, double dt, double* adj) 
     const int ds = (n % 2 == 0) ? ds0 : !ds0;
     const int nd = split_dirs(c, ds, dirs);
     for (int k = nd - 1; k >= 0; k--) {
-      const double* in = c->a_states + (int64_t)((n - n0) * nd + k) * el;
+      const double* in = states + (int64_t)((n - n0) * nd + k) * el;
       int rc = sweep_adjoint(c, in, cur, oth, dt, dirs[k], false);
       if (rc != AWFL_OK) return rc;
       std::swap(cur, oth);
@@ -993,15 +993,12 @@ int reverse_steps(awfl_ctx* c, int n0, int n1, int ds0, double dt, double* adj) 
   return AWFL_OK;
 }
 
-// split inputs of steps [n0, n1) from the state before step n0
-int record_steps(awfl_ctx* c, const double* start, int n0, int n1, int ds0, double dt) {
+// split inputs of steps [n0, n1) into `states`, from the state before step n0
+int record_into(awfl_ctx* c, double* states, const double* start, int n0, int n1, int ds0, double dt) {
   const int64_t el = c->state_elems();
-  c->a_rec_ckpt = nullptr;             // a_states is rewritten
   int dirs[3];
   const int nd = split_dirs(c, 1, dirs);
-  int rc = ensure_states(c, (int64_t)(n1 - n0) * nd);
-  if (rc != AWFL_OK) return rc;
-  HIP_TRY(c, hipMemcpyAsync(c->a_states, start, sizeof(double) * el, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(states, start, sizeof(double) * el, hipMemcpyDeviceToDevice, c->stream));
   const int64_t last = (int64_t)(n1 - n0) * nd - 1;
   for (int n = n0; n < n1; n++) {
     const int ds = (n % 2 == 0) ? ds0 : !ds0;
@@ -1009,12 +1006,45 @@ int record_steps(awfl_ctx* c, const double* start, int n0, int n1, int ds0, doub
     for (int k = 0; k < nd; k++) {
       const int64_t idx = (int64_t)(n - n0) * nd + k;
       if (idx == last) break;
-      rc = awfl_sweep(c, c->a_states + idx * el, c->a_states + (idx + 1) * el, dt, dirs[k]);
+      int rc = awfl_sweep(c, states + idx * el, states + (idx + 1) * el, dt, dirs[k]);
       if (rc != AWFL_OK) return rc;
     }
   }
   return AWFL_OK;
 }
+
+// the same into c->a_states, on the context's stream
+int record_steps(awfl_ctx* c, const double* start, int n0, int n1, int ds0, double dt) {
+  c->a_rec_ckpt = nullptr;             // a_states is rewritten
+  int dirs[3];
+  const int nd = split_dirs(c, 1, dirs);
+  int rc = ensure_states(c, (int64_t)(n1 - n0) * nd);
+  if (rc != AWFL_OK) return rc;
+  return record_into(c, c->a_states, start, n0, n1, ds0, dt);
+}
+
+// the second split-input buffer, the recomputation stream and its events
+int ensure_pipeline(awfl_ctx* c, int64_t nstates) {
+  if (c->a_states2_n < nstates) {
+    if (c->a_states2) HIP_TRY(c, hipFree(c->a_states2));
+    c->a_states2 = nullptr;
+    c->a_states2_n = 0;
+    HIP_TRY(c, hipMalloc(&c->a_states2, sizeof(double) * c->state_elems() * nstates));
+    c->a_states2_n = nstates;
+  }
+  if (!c->a_s2) HIP_TRY(c, hipStreamCreateWithFlags(&c->a_s2, hipStreamNonBlocking));
+  for (hipEvent_t& e : c->a_ev)
+    if (!e) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return AWFL_OK;
+}
+
+// launches of the context go to another stream while it lives
+struct StreamSwap {
+  awfl_ctx* c;
+  hipStream_t saved;
+  StreamSwap(awfl_ctx* ctx, hipStream_t s) : c(ctx), saved(ctx->stream) { c->stream = s; }
+  ~StreamSwap() { c->stream = saved; }
+};
 
 }  // namespace
 
@@ -1063,7 +1093,7 @@ int awfl_time_step_adjoint(awfl_ctx* c, const double* d_state_before, double* d_
   if (rc != AWFL_OK) return rc;
   rc = record_steps(c, d_state_before, 0, 1, dim_switch ? 1 : 0, dt);
   if (rc != AWFL_OK) return rc;
-  return reverse_steps(c, 0, 1, dim_switch ? 1 : 0, dt, d_adj);
+  return reverse_steps(c, c->a_states, 0, 1, dim_switch ? 1 : 0, dt, d_adj);
 }
 
 int awfl_forward_ckpt(awfl_ctx* c, double* d_state, double dt, int nsteps, int every, double* d_ckpt) {
@@ -1087,11 +1117,15 @@ int awfl_forward_ckpt(awfl_ctx* c, double* d_state, double dt, int nsteps, int e
   if (nsteps == 0) return AWFL_OK;
   HIP_TRY(c, hipMemcpyAsync(d_ckpt + (int64_t)(nseg - 1) * el, d_state, sizeof(double) * el,
                             hipMemcpyDeviceToDevice, c->stream));
-  int rc = record_steps(c, d_state, nlast, nsteps, ds0, dt);
+  // room for a whole segment, as the adjoint needs, so that it keeps these
+  int dirs[3];
+  const int nd = split_dirs(c, 1, dirs);
+  int rc = ensure_states(c, (int64_t)every * nd);
+  if (rc != AWFL_OK) return rc;
+  rc = record_steps(c, d_state, nlast, nsteps, ds0, dt);
   if (rc != AWFL_OK) return rc;
   // the last sweep of the last step, from its recorded input into the state
-  int dirs[3];
-  const int nd = split_dirs(c, ((nsteps - 1) % 2 == 0) ? ds0 : !ds0, dirs);
+  split_dirs(c, ((nsteps - 1) % 2 == 0) ? ds0 : !ds0, dirs);
   const int64_t last = (int64_t)(nsteps - nlast) * nd - 1;
   rc = awfl_sweep(c, c->a_states + last * el, d_state, dt, dirs[nd - 1]);
   if (rc != AWFL_OK) return rc;
@@ -1122,14 +1156,43 @@ int awfl_adjoint_ckpt(awfl_ctx* c, const double* d_ckpt, double dt, int nsteps, 
   const bool kept = c->a_rec_ckpt == d_ckpt && c->a_rec_dt == dt && c->a_rec_nsteps == nsteps &&
                     c->a_rec_every == every && c->a_rec_ds0 == ds0;
   c->a_rec_ckpt = nullptr;
+  if (nseg == 0) return AWFL_OK;
+  int dirs[3];
+  const int nd = split_dirs(c, 1, dirs);
+  const int64_t nst = (int64_t)every * nd;
+  rc = ensure_states(c, nst);            // keeps a_states when it is large enough
+  if (rc == AWFL_OK && nseg > 1) rc = ensure_pipeline(c, nst);
+  if (rc != AWFL_OK) return rc;
+  // segment m's split inputs are in buf[m % 2 == (nseg - 1) % 2 ? 0 : 1]; the
+  // last segment's in a_states (kept from the forward run, or recomputed
+  // here).  While the sweep adjoints of segment m run on the context's
+  // stream, segment m-1 is recomputed into the other buffer on a_s2: after
+  // everything enqueued so far on the context's stream (the reverse of
+  // segment m+1 read that buffer last), and the reverse of segment m-1 waits
+  // for it.
+  double* buf[2] = {c->a_states, c->a_states2};
+  int cur = 0;
+  if (!kept) {
+    const int m = nseg - 1;
+    rc = record_into(c, buf[cur], d_ckpt + (int64_t)m * el, m * every, std::min(nsteps, m * every + every), ds0, dt);
+    if (rc != AWFL_OK) return rc;
+  }
   for (int m = nseg - 1; m >= 0; m--) {
     const int n0 = m * every, n1 = std::min(nsteps, n0 + every);
-    if (!(kept && m == nseg - 1)) {
-      rc = record_steps(c, d_ckpt + (int64_t)m * el, n0, n1, ds0, dt);
+    if (m > 0) {
+      HIP_TRY(c, hipEventRecord(c->a_ev[0], c->stream));
+      HIP_TRY(c, hipStreamWaitEvent(c->a_s2, c->a_ev[0], 0));
+      {
+        StreamSwap sw(c, c->a_s2);
+        rc = record_into(c, buf[1 - cur], d_ckpt + (int64_t)(m - 1) * el, n0 - every, n0, ds0, dt);
+      }
       if (rc != AWFL_OK) return rc;
+      HIP_TRY(c, hipEventRecord(c->a_ev[1], c->a_s2));
     }
-    rc = reverse_steps(c, n0, n1, ds0, dt, d_adj);
+    rc = reverse_steps(c, buf[cur], n0, n1, ds0, dt, d_adj);
     if (rc != AWFL_OK) return rc;
+    if (m > 0) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->a_ev[1], 0));
+    cur = 1 - cur;
   }
   return AWFL_OK;
 }

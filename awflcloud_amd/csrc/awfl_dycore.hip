@@ -855,8 +855,11 @@ void awfl_destroy(awfl_ctx* c) {
   if (c->d_bgf) (void)hipFree(c->d_bgf);
   for (float* b : c->f_buf)
     if (b) (void)hipFree(b);
-  for (double* b : {c->a_scr, c->a_tmp, c->a_states, c->r_tend[0], c->r_tend[1], c->t_buf[0], c->t_buf[1]})
+  for (double* b : {c->a_scr, c->a_tmp, c->a_states, c->a_states2, c->r_tend[0], c->r_tend[1], c->t_buf[0], c->t_buf[1]})
     if (b) (void)hipFree(b);
+  for (hipEvent_t e : c->a_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->a_s2) (void)hipStreamDestroy(c->a_s2);
   delete c;
 }
 

@@ -121,6 +121,14 @@ struct awfl_ctx {
   const double* a_rec_ckpt = nullptr;
   double a_rec_dt = 0;
   int a_rec_nsteps = 0, a_rec_every = 0, a_rec_ds0 = 0;
+  // the checkpointed adjoint recomputes the next segment's split inputs into
+  // a_states2 on a stream of its own (a_s2, default priority) while the
+  // current segment's sweep adjoints run on the context's stream; a_ev
+  // orders the two
+  double* a_states2 = nullptr;
+  int64_t a_states2_n = 0;
+  hipStream_t a_s2 = nullptr;
+  hipEvent_t a_ev[2] = {nullptr, nullptr};
   // alternative integrators (awfl_splitting.hip): two tendency-sized arrays
   double* r_tend[2] = {nullptr, nullptr};
   // tracers (awfl_tracers.hip): count, positivity bit mask, ping-pong arrays
