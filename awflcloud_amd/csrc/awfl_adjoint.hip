@@ -520,10 +520,30 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
     });
   }
   __syncthreads();
-  // 3. faces: flux adjoint -> edge-state adjoints, over the edge states
+  // 3. faces: flux adjoint -> edge-state adjoints, over the edge states.  A
+  // z line with walls has n + 1 faces for n threads: thread 0 takes both wall
+  // faces through the short wall form (riemann_adj_zwall), the others one
+  // interior face each, all in one pass.
   const double cdr = K.dt * K.rd;
-  const int nfaces = per ? n : n + 1;
+  const bool zwall = DIR == 2 && !per;
+  const int nfaces = (per || zwall) ? n : n + 1;
   for (int f = threadIdx.x; f < nfaces; f += bd) {
+    if (zwall && f == 0) {
+      sfor<2>([&](auto side) {         // face 0, then face n
+        const int sl = side == 0 ? 0 : 5 * n + n - 1;            // the boundary cell's edge slot
+        const int ca = side == 0 ? 0 : n - 1;
+        double L[5], Fb[5], Lb[5];
+        sfor<5>([&](auto l) {
+          L[l] = s_e[sl + l * n];
+          const double v = side == 0 ? s_ao[l * n + ca] : -s_ao[l * n + ca];
+          Fb[l] = (G.sim2d && l == idV) ? 0.0 : cdr * v;
+          Lb[l] = 0.0;
+        });
+        adj::riemann_adj_zwall(L, fref_of(side == 0 ? 0 : n), Fb, Lb);
+        sfor<5>([&](auto l) { s_e[sl + l * n] = Lb[l]; });
+      });
+      continue;
+    }
     const bool wl = !per && f == 0, wr = !per && f == n;
     const int a = wl ? 0 : (f == 0 ? n - 1 : f - 1);      // cell on the low side
     const int b = wr ? n - 1 : (f == n ? 0 : f);          // cell on the high side

@@ -388,6 +388,20 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const FaceR
   }
 }
 
+// Lb += (dF(L, L) / dL)^T Fb at a solid z wall, whose face takes both sides
+// from the boundary cell's edge state L (S:1577-1578) and where the wall rows
+// of the transform make the normal momentum L[idW] zero.  A consistent flux
+// of two equal states is the physical flux (rho w, rho u w, rho v w,
+// rho w^2 + p, rho theta w) / rho of L, whose derivative at rho w = 0 is
+// nonzero only in d(rho w) and in dp/d(rho theta) = gamma p / (rho theta):
+// the same derivative riemann_adj forms term by term, in a few operations.
+AWFL_HD void riemann_adj_zwall(const double (&L)[5], const FaceRef& fr, const double (&Fb)[5], double (&Lb)[5]) {
+  const double iL = rcp(L[0]);
+  const double p5 = C0 * pow_at(L[4], fr.q5);
+  Lb[idW] += Fb[0] + (Fb[idU] * L[idU] + Fb[idV] * L[idV] + Fb[idT] * L[idT]) * iL;
+  Lb[idT] += Fb[idW] * GAMMA * p5 * rcp(L[4]);
+}
+
 // tvb * d tv_eval<N>(a) / da, accumulated into ab
 template <int N, int M>
 AWFL_HD void tv_adj(const double (&a)[M], double tvb, double (&ab)[M]) {
