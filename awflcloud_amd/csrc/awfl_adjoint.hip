@@ -1046,6 +1046,7 @@ int record_steps(awfl_ctx* c, const double* start, int n0, int n1, int ds0, doub
 // the second split-input buffer, the recomputation stream and its events
 int ensure_pipeline(awfl_ctx* c, int64_t nstates) {
   if (c->a_states2_n < nstates) {
+    c->a_rec_ckpt = nullptr;
     if (c->a_states2) HIP_TRY(c, hipFree(c->a_states2));
     c->a_states2 = nullptr;
     c->a_states2_n = 0;
@@ -1121,13 +1122,16 @@ int awfl_forward_ckpt(awfl_ctx* c, double* d_state, double dt, int nsteps, int e
   if (!d_state || !d_ckpt) return fail(c, AWFL_E_ARG, "null buffer");
   if (nsteps < 0 || every < 1) return fail(c, AWFL_E_ARG, "nsteps >= 0 and every >= 1");
   const int64_t el = c->state_elems();
-  // the last segment runs as the split sweeps the adjoint would recompute,
-  // keeping their inputs (a_states) for awfl_adjoint_ckpt on the same
-  // checkpoints; the earlier segments run as fused steps
+  // the last two segments run as the split sweeps the adjoint would
+  // recompute, keeping their inputs for awfl_adjoint_ckpt on the same
+  // checkpoints (the last in a_states, the one before in a_states2: the two
+  // buffers the adjoint pipelines through anyway); the earlier segments run
+  // as fused steps
   const int ds0 = c->dimSwitch ? 1 : 0;
   const int nseg = (nsteps + every - 1) / every;
-  const int nlast = nsteps > 0 ? (nseg - 1) * every : 0;
-  for (int n = 0; n < nlast; n++) {
+  const int nkeep = std::min(2, nseg);
+  const int nfused = (nseg - nkeep) * every;
+  for (int n = 0; n < nfused; n++) {
     if (n % every == 0)
       HIP_TRY(c, hipMemcpyAsync(d_ckpt + (int64_t)(n / every) * el, d_state, sizeof(double) * el,
                                 hipMemcpyDeviceToDevice, c->stream));
@@ -1135,26 +1139,33 @@ int awfl_forward_ckpt(awfl_ctx* c, double* d_state, double dt, int nsteps, int e
     if (rc != AWFL_OK) return rc;
   }
   if (nsteps == 0) return AWFL_OK;
-  HIP_TRY(c, hipMemcpyAsync(d_ckpt + (int64_t)(nseg - 1) * el, d_state, sizeof(double) * el,
-                            hipMemcpyDeviceToDevice, c->stream));
-  // room for a whole segment, as the adjoint needs, so that it keeps these
+  // room for whole segments, as the adjoint needs, so that it keeps these
   int dirs[3];
   const int nd = split_dirs(c, 1, dirs);
   int rc = ensure_states(c, (int64_t)every * nd);
+  if (rc == AWFL_OK && nkeep == 2) rc = ensure_pipeline(c, (int64_t)every * nd);
   if (rc != AWFL_OK) return rc;
-  rc = record_steps(c, d_state, nlast, nsteps, ds0, dt);
-  if (rc != AWFL_OK) return rc;
-  // the last sweep of the last step, from its recorded input into the state
-  split_dirs(c, ((nsteps - 1) % 2 == 0) ? ds0 : !ds0, dirs);
-  const int64_t last = (int64_t)(nsteps - nlast) * nd - 1;
-  rc = awfl_sweep(c, c->a_states + last * el, d_state, dt, dirs[nd - 1]);
-  if (rc != AWFL_OK) return rc;
+  c->a_rec_ckpt = nullptr;             // the kept inputs are rewritten
+  for (int m = nseg - nkeep; m < nseg; m++) {
+    const int n0 = m * every, n1 = std::min(nsteps, n0 + every);
+    double* buf = m == nseg - 1 ? c->a_states : c->a_states2;
+    HIP_TRY(c, hipMemcpyAsync(d_ckpt + (int64_t)m * el, d_state, sizeof(double) * el, hipMemcpyDeviceToDevice,
+                              c->stream));
+    rc = record_into(c, buf, d_state, n0, n1, ds0, dt);
+    if (rc != AWFL_OK) return rc;
+    // the last sweep of the segment's last step, from its recorded input into the state
+    split_dirs(c, ((n1 - 1) % 2 == 0) ? ds0 : !ds0, dirs);
+    const int64_t last = (int64_t)(n1 - n0) * nd - 1;
+    rc = awfl_sweep(c, buf + last * el, d_state, dt, dirs[nd - 1]);
+    if (rc != AWFL_OK) return rc;
+  }
   c->dimSwitch = (nsteps % 2 == 0) ? ds0 : !ds0;   // as nsteps fused steps leave it
   c->a_rec_ckpt = d_ckpt;
   c->a_rec_dt = dt;
   c->a_rec_nsteps = nsteps;
   c->a_rec_every = every;
   c->a_rec_ds0 = ds0;
+  c->a_rec_kept = nkeep;
   return AWFL_OK;
 }
 
@@ -1184,22 +1195,24 @@ int awfl_adjoint_ckpt(awfl_ctx* c, const double* d_ckpt, double dt, int nsteps, 
   if (rc == AWFL_OK && nseg > 1) rc = ensure_pipeline(c, nst);
   if (rc != AWFL_OK) return rc;
   // segment m's split inputs are in buf[m % 2 == (nseg - 1) % 2 ? 0 : 1]; the
-  // last segment's in a_states (kept from the forward run, or recomputed
-  // here).  While the sweep adjoints of segment m run on the context's
-  // stream, segment m-1 is recomputed into the other buffer on a_s2: after
-  // everything enqueued so far on the context's stream (the reverse of
-  // segment m+1 read that buffer last), and the reverse of segment m-1 waits
-  // for it.
+  // last segment's in a_states and the one before in a_states2 (kept from
+  // the forward run, or recomputed here).  While the sweep adjoints of
+  // segment m run on the context's stream, segment m-1 is recomputed into
+  // the other buffer on a_s2: after everything enqueued so far on the
+  // context's stream (the reverse of segment m+1 read that buffer last), and
+  // the reverse of segment m-1 waits for it.
+  const int nkept = kept ? c->a_rec_kept : 0;
   double* buf[2] = {c->a_states, c->a_states2};
   int cur = 0;
-  if (!kept) {
+  if (nkept < 1) {
     const int m = nseg - 1;
     rc = record_into(c, buf[cur], d_ckpt + (int64_t)m * el, m * every, std::min(nsteps, m * every + every), ds0, dt);
     if (rc != AWFL_OK) return rc;
   }
   for (int m = nseg - 1; m >= 0; m--) {
     const int n0 = m * every, n1 = std::min(nsteps, n0 + every);
-    if (m > 0) {
+    const bool recompute = m > 0 && m - 1 < nseg - nkept;   // segment m-1 not kept
+    if (recompute) {
       HIP_TRY(c, hipEventRecord(c->a_ev[0], c->stream));
       HIP_TRY(c, hipStreamWaitEvent(c->a_s2, c->a_ev[0], 0));
       {
@@ -1211,7 +1224,7 @@ int awfl_adjoint_ckpt(awfl_ctx* c, const double* d_ckpt, double dt, int nsteps, 
     }
     rc = reverse_steps(c, buf[cur], n0, n1, ds0, dt, d_adj);
     if (rc != AWFL_OK) return rc;
-    if (m > 0) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->a_ev[1], 0));
+    if (recompute) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->a_ev[1], 0));
     cur = 1 - cur;
   }
   return AWFL_OK;

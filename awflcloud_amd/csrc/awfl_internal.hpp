@@ -121,6 +121,7 @@ struct awfl_ctx {
   const double* a_rec_ckpt = nullptr;
   double a_rec_dt = 0;
   int a_rec_nsteps = 0, a_rec_every = 0, a_rec_ds0 = 0;
+  int a_rec_kept = 0;               // trailing segments kept: the last in a_states, the one before in a_states2
   // the checkpointed adjoint recomputes the next segment's split inputs into
   // a_states2 on a stream of its own (a_s2, default priority) while the
   // current segment's sweep adjoints run on the context's stream; a_ev
