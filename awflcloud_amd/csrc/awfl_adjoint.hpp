@@ -118,10 +118,12 @@ AWFL_HD void cell_dt(const T (&q0)[5][NGLL], const CellBg<NGLL>& bg, const Sweep
   }
   T sv = T(0.0);
   sfor<NGLL>([&](auto ii) {
-    tp.rr[ii] = rcp_g(q[idR][0][ii]);
+    // 1/rho and 1/(rho theta) from one reciprocal of their product
+    const T z = rcp_g(q[idR][0][ii] * q[idT][0][ii]);
+    tp.rr[ii] = q[idT][0][ii] * z;
+    tp.rrt[ii] = q[idR][0][ii] * z;
     sfor<4>([&](auto m) { f[m][0][ii] = q[NM][0][ii] * q[m + 1][0][ii] * tp.rr[ii]; });
     g[0][ii] = pow_at(q[idT][0][ii], DIR == 2 ? bg.ptG[ii] : bg.pt);
-    tp.rrt[ii] = rcp_g(q[idT][0][ii]);
     if constexpr (DIR == 2) sv = -(q[idR][0][ii] - T(bg.rG[ii])) * T(GRAV);
   });
   if constexpr (DIR == 2) {
@@ -301,11 +303,13 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const FaceR
   }
   double rt = r * t;
   double p = C0 * pow_at(rt, fr.rt);
-  double ir = rcp(r);
+  // 1/r, 1/t and 1/(r t) from one reciprocal
+  const double irt = rcp(rt);
+  double ir = t * irt;
   double cs2 = GAMMA * p * ir;
   double cs, i2c;
   sqrt_halfrsqrt(cs2, cs, i2c);     // as the forward riemann
-  double it = rcp(t);
+  double it = r * irt;
   double ics = 2.0 * i2c;
   const bool up = un > 0;
   const double(&U)[5] = up ? L : R;
@@ -321,7 +325,9 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const FaceR
   double qt1 = w2 + ut1 * s56;
   double qt2 = w3 + ut2 * s56;
   double q5 = t * s56;
-  double iq1 = rcp(q1);
+  // 1/q1 and 1/q5 from one reciprocal
+  const double iq15 = rcp(q1 * q5);
+  double iq1 = q5 * iq15;
   double p5 = C0 * pow_at(q5, fr.q5);
   // reverse
   double qnb = Fb[0] + Fb[NM] * 2.0 * qn * iq1 + Fb[T1] * qt1 * iq1 + Fb[T2] * qt2 * iq1 + Fb[4] * q5 * iq1;
@@ -329,7 +335,7 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const FaceR
   double q1b = -iq1b * iq1 * iq1;
   double qt1b = Fb[T1] * qn * iq1;
   double qt2b = Fb[T2] * qn * iq1;
-  double q5b = Fb[NM] * GAMMA * p5 * rcp(q5) + Fb[4] * qn * iq1;
+  double q5b = Fb[NM] * GAMMA * p5 * (q1 * iq15) + Fb[4] * qn * iq1;
   double tb = q5b * s56;
   double s56b = q5b * t + qt1b * ut1 + qt2b * ut2 + q1b;
   double w2b = qt1b, w3b = qt2b;
@@ -365,7 +371,7 @@ AWFL_HD void riemann_adj(const double (&L)[5], const double (&R)[5], const FaceR
   double cs2b = csb * 0.5 * ics;
   double pb = cs2b * GAMMA * ir;
   double rb = -cs2b * cs2 * ir;
-  double rtb = pb * GAMMA * p * rcp(rt);
+  double rtb = pb * GAMMA * p * irt;
   rb += rtb * t;
   tb += rtb * r;
   // averages
@@ -476,21 +482,24 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
   // the NP reciprocals 1/(tv^2 + eps) and 1/(d^2 + w1 (1 - 2d)) each from
   // one reciprocal of the product and the products of the others
   double rDn[NP], b[NP], w1[NP], m[NP], w[NP], rden[NP];
-  double S1 = WENO_EPS;
+  double R1, R2;
   {
     double D[NP], ex[NP], P;
     sfor<NP>([&](auto i) { D[i] = tv[i] * tv[i] + WENO_EPS; });
     excl_products<NP>(D, ex, P);
-    const double rP = rcp(P);
+    // 1/P and 1/S1 = P / (eps P + sum_i d_i ex_i) from one reciprocal
+    double N = 0.0;
+    sfor<NP>([&](auto i) { N += W::idl[i] * ex[i]; });
+    const double T1 = fma(WENO_EPS, P, N);
+    const double z = rcp(P * T1);
+    const double rP = T1 * z;
+    R1 = P * P * z;
     sfor<NP>([&](auto i) {
       constexpr double d = W::idl[i];
       rDn[i] = ex[i] * rP;
       b[i] = d * rDn[i];
-      S1 += b[i];
     });
   }
-  const double R1 = rcp(S1);
-  double S2 = WENO_EPS;
   {
     double E[NP], ex[NP], P;
     sfor<NP>([&](auto i) {
@@ -499,15 +508,22 @@ AWFL_HD void reconstruct_adj(const double (&u)[ORD], bool doweno, const double (
       E[i] = d * d + w1[i] * (1.0 - 2.0 * d);
     });
     excl_products<NP>(E, ex, P);
-    const double rP = rcp(P);
+    // 1/P and 1/S2 from one reciprocal, as above
+    double A[NP], M = 0.0;
     sfor<NP>([&](auto i) {
       constexpr double d = W::idl[i];
+      A[i] = w1[i] * (d + d * d - 3.0 * d * w1[i] + w1[i] * w1[i]);
+      M += A[i] * ex[i];
+    });
+    const double T2 = fma(WENO_EPS, P, M);
+    const double z = rcp(P * T2);
+    const double rP = T2 * z;
+    R2 = P * P * z;
+    sfor<NP>([&](auto i) {
       rden[i] = ex[i] * rP;
-      m[i] = w1[i] * (d + d * d - 3.0 * d * w1[i] + w1[i] * w1[i]) * rden[i];
-      S2 += m[i];
+      m[i] = A[i] * rden[i];
     });
   }
-  const double R2 = rcp(S2);
   sfor<NP>([&](auto i) { w[i] = m[i] * R2; });
   // reverse: GLL projection (the centre cell's average takes the sum of the
   // GLL adjoints), weighted sum
