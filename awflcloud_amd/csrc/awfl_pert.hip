@@ -423,8 +423,11 @@ int launch_sweep_f32(awfl_ctx* c, const float* s, const float* cs, float* so, fl
   int n = dir == 1 ? c->cfg.ny : c->cfg.nz;
   int other = dir == 1 ? c->cfg.nz : c->cfg.ny;
   int bx = (c->cfg.nx + 63) / 64;
+  // segments as in the FP64 sweeps: at most 32 cells, down to 16 for ~4
+  // waves per SIMD
   int64_t layer = (int64_t)bx * other;
-  int nseg = (int)std::max<int64_t>(1, std::min<int64_t>((4096 + layer - 1) / layer, std::max(1, n / 16)));
+  int nseg = (int)std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>((4096 + layer - 1) / layer, (n + 31) / 32),
+                                                           std::max(1, n / 16)));
   int seglen = (n + nseg - 1) / nseg;
   nseg = (n + seglen - 1) / seglen;
   dim3 grid(bx, other, nseg);
