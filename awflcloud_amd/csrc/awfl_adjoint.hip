@@ -627,9 +627,11 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
   // drops it when it falls outside the line.  At a wall the stencil slots
   // past the end read the clamped end cell; the normal momentum there is
   // zero, so it passes nothing back.
-  const int wrap = per ? n : 0;
-  auto gather = [&](int l0, auto nv, auto small) {
+  // MODE 0: periodic with n < hs (wrap by the remainder), 1: periodic with
+  // n >= hs (one period at most, every slot in the line), 2: walls
+  auto gather = [&](int l0, auto nv, auto mode) {
     constexpr int NV = decltype(nv)::value;
+    constexpr int MODE = decltype(mode)::value;
     for (int c = threadIdx.x; c < n; c += bd) {
       sfor<NV>([&](auto v) {
         const int l = l0 + v;
@@ -639,19 +641,23 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
             acc += s_r[adj_row<ORD, NGLL>(l, s) * n + c];
           } else {
             int src = c + hs - s;
-            if constexpr (decltype(small)::value) {
+            if constexpr (MODE == 0) {
               src %= n;
               src += src < 0 ? n : 0;
-            } else {
-              src += src < 0 ? wrap : 0;
-              src -= src >= n ? wrap : 0;
+            } else if constexpr (MODE == 1) {
+              src += src < 0 ? n : 0;
+              src -= src >= n ? n : 0;
             }
-            const bool ok = src >= 0 && src < n;
-            const double val = s_r[adj_row<ORD, NGLL>(l, s) * n + (ok ? src : c)];
-            acc += ok ? val : 0.0;
+            if constexpr (MODE != 2) {
+              acc += s_r[adj_row<ORD, NGLL>(l, s) * n + src];
+            } else {
+              const bool ok = src >= 0 && src < n;
+              const double val = s_r[adj_row<ORD, NGLL>(l, s) * n + (ok ? src : c)];
+              acc += ok ? val : 0.0;
+            }
           }
         });
-        if (!per && l != NM) {
+        if (MODE == 2 && l != NM) {
           if (c == 0)                    // slot s of cell cc reads index cc - hs + s < 0
             sfor<hs>([&](auto cc) {
               sfor<ORD>([&](auto s) {
@@ -674,8 +680,9 @@ __global__ __launch_bounds__(256) void k_adj_line(const double* __restrict__ in,
       });
     }
   };
-  if (per && n < hs) gather(0, std::integral_constant<int, 5>{}, std::true_type{});
-  else gather(0, std::integral_constant<int, 5>{}, std::false_type{});
+  if (!per) gather(0, std::integral_constant<int, 5>{}, std::integral_constant<int, 2>{});
+  else if (n < hs) gather(0, std::integral_constant<int, 5>{}, std::integral_constant<int, 0>{});
+  else gather(0, std::integral_constant<int, 5>{}, std::integral_constant<int, 1>{});
   // the halo cells of an x line (y and z halos: k_zero_halo)
   if constexpr (DIR == 0) {
     for (int c = threadIdx.x; c < 2 * hs; c += bd) {
