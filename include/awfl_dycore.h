@@ -228,21 +228,25 @@ int awfl_time_step_adjoint(awfl_ctx* ctx, const double* d_state_before, double* 
 /* nsteps awfl_time_step calls in place on d_state, copying the state before
  * steps 0, every, 2*every, ... into consecutive state-sized slots of d_ckpt
  * (ceil(nsteps/every) slots).  Read awfl_get_dim_switch first: the adjoint
- * run needs the value it had before the first step.  The last segment runs
- * as its split sweeps and keeps their inputs in the context (device memory
- * for every*3 states), which the next awfl_adjoint_ckpt on the same d_ckpt
- * with the same dt, nsteps, every and dim_switch0 uses instead of
- * recomputing them; the state it produces is the same, bit for bit. */
+ * run needs the value it had before the first step.  The last two segments
+ * run as their split sweeps and keep their inputs in the context (device
+ * memory for 2*every*3 states, the two buffers awfl_adjoint_ckpt pipelines
+ * through), which the next awfl_adjoint_ckpt on the same d_ckpt with the
+ * same dt, nsteps, every and dim_switch0 uses instead of recomputing them;
+ * the state it produces is the same, bit for bit. */
 int awfl_forward_ckpt(awfl_ctx* ctx, double* d_state, double dt, int nsteps, int every, double* d_ckpt);
 
 /* Adjoint of the run of awfl_forward_ckpt: d_adj enters as dJ/d(final
  * state) and leaves as dJ/d(initial state).  Segment by segment, last first,
  * the split inputs of the segment's steps are recomputed from its checkpoint
- * (device memory for every*3 states) and the split adjoints applied in
- * reverse order; the last segment's inputs are taken as awfl_forward_ckpt
- * left them when it ran on the same checkpoints and arguments, on the same
- * stream, immediately before (the checkpoint contents must be unchanged
- * since).  They serve one adjoint: any later adjoint recomputes them. */
+ * and the split adjoints applied in reverse order.  Two buffers of every*3
+ * states alternate: the next segment is recomputed into one on a second
+ * stream of the context (default priority, ordered by events) while the
+ * adjoints of the current one run on the context's stream.  The inputs of
+ * the last two segments are taken as awfl_forward_ckpt left them when it ran
+ * on the same checkpoints and arguments, on the same stream, immediately
+ * before (the checkpoint contents must be unchanged since).  They serve one
+ * adjoint: any later adjoint recomputes them. */
 int awfl_adjoint_ckpt(awfl_ctx* ctx, const double* d_ckpt, double dt, int nsteps, int every, int dim_switch0,
                       double* d_adj);
 
