@@ -413,6 +413,573 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   sweep_tr<ORD, NGLL, 2, TEND>(in, tr_in, out, tr_out, G, K, ntr, pos);
 }
 
+// ---------------------------------------------------------------------------
+// y and z sweeps: the tracer walker.
+//
+// A y or z line is strided in memory (each cell on its own cache line), so
+// the line kernel above, one line per work-group, issues one cache line per
+// lane for every load and store.  Here a work-group of four waves takes
+// WK_WX = 4 lines that are neighbours in x, wave w on line w, and walks them
+// in steps of WK_SEG = 64 cells, lane c on cell X = c0 + c of the step.
+// - Loads are issued with the lanes across x (the four lines' values of one
+//   cell are one 32-byte run) into a ring of each line's cells in LDS: the
+//   state, the mixing ratios, the tracer masses and (z) the cell sources.
+//   The ring holds the step's stencils, cells c0 - HB .. c0 + 63 + hs.
+// - A lane forms cell X's edges (the production cell_edges and the tracer
+//   transform of the line kernel), then face X, the left face of its cell,
+//   from the right edge of cell X - 1 (the next lower lane by a shuffle, or
+//   the last cell of the previous step kept in LDS).
+// - The positivity limiter needs a cell's factor from its two faces' fluxes
+//   and a face's factor from its upwind cell, so a lane forms the factor of
+//   cell X - 1, limits face X - 1 and updates cell X - 2: the update trails
+//   the faces by two cells.  What the next step's first lanes need of the
+//   last cells of this one is kept in a small carry area.
+// - Cells 0 and 1 need face 0, which a periodic line forms from the last
+//   cell's edge (and the limiter of either boundary may take the factor of
+//   the cell at the other end, as the line kernel's indexing does), so the
+//   first step keeps what they need in a head area and a tail pass after the
+//   last step forms faces 0 and n and updates cells n-2, n-1, 0 and 1.
+// Every per-line exchange stays inside the line's wave; the work-group meets
+// at barriers only around the ring's loads.  The arithmetic of every cell,
+// face, factor and update is the line kernel's, statement for statement.
+// ---------------------------------------------------------------------------
+constexpr int WK_SEG = 64;   // cells of a line per step, one per lane
+constexpr int WK_WX = 4;     // lines per work-group, one per wave
+
+template <int ORD>
+struct WalkGeom {
+  static constexpr int hs = (ORD - 1) / 2;
+  static constexpr int HB = hs > 2 ? hs : 2;       // ring cells behind a step's first cell
+  static constexpr int R = WK_SEG + HB + hs;       // ring length
+};
+
+// per-line LDS layout, in doubles
+struct WalkLds {
+  int ring;   // rows of R: state [5], mixing ratio [ntr], mass [ntr], (z) source [1]
+  int carry;  // of the step's last cell X: right edge [5], F(X) [5], F(X-1) [5], limiter direction [1],
+              // then per tracer its high edge, flux of face X, factor of cell X-1, limited flux of face X-1
+  int head;   // cells 0 and 1 (first step), for the tail
+  int tail;   // faces 0 and n, factors of cells 0 and n-1, limited faces n-1, 0, n, 1, n-2
+  int size;
+};
+// head offsets
+constexpr int WH_EL0 = 0, WH_F1 = 5, WH_F2 = 10, WH_DIR1 = 15, WH_TR = 16;   // then tlo0, TF1, fac1, TFl2 [ntr] each, then two cells
+__host__ __device__ inline WalkLds walk_lds(int ntr, int R, bool z) {
+  WalkLds L{};
+  int o = 0;
+  L.ring = o;  o += (5 + 2 * ntr + (z ? 1 : 0)) * R;
+  L.carry = o; o += 16 + 4 * ntr;
+  L.head = o;  o += WH_TR + 4 * ntr + 2 * (6 + ntr);
+  L.tail = o;  o += 12 + 9 * ntr;
+  L.size = o;
+  return L;
+}
+
+// orders one wave's LDS accesses across its lanes (a lane reads what another
+// lane of the same wave wrote): no instruction, a fence for the compiler
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The walker's arguments, one struct at the start of the kernel argument
+// segment.  Each part of the walk reads them again through a pointer the
+// compiler cannot follow (walk_args): loaded once before the walk's loop,
+// the sweep constants and grid fields stay live through every step's cells
+// (registers and scalar registers spilled), where each part now loads what
+// it uses, as the line kernel does.
+struct WalkArgs {
+  const double* in;
+  const double* tr_in;
+  double* out;
+  double* tr_out;
+  Grid G;
+  SweepK K;
+  int ntr;
+  unsigned pos;
+};
+__device__ __forceinline__ WalkArgs walk_args() {
+#if __HIP_DEVICE_COMPILE__
+  uint64_t v = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(v));
+  const __attribute__((address_space(4))) WalkArgs* p = (const __attribute__((address_space(4))) WalkArgs*)v;
+  WalkArgs r;
+  r.in = p->in;
+  r.tr_in = p->tr_in;
+  r.out = p->out;
+  r.tr_out = p->tr_out;
+  r.G = p->G;
+  r.K = p->K;
+  r.ntr = p->ntr;
+  r.pos = p->pos;
+  return r;
+#else
+  return WalkArgs{};   // host pass: never called
+#endif
+}
+
+// one part of the walk: 0 the first step's ring and the carries, 1 step s,
+// 2 the ring loads of step s, 3 the tail
+template <int ORD, int NGLL, int DIR, bool TEND>
+__device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
+  static_assert(DIR == 1 || DIR == 2, "the walker runs the y and z sweeps");
+  const double* __restrict__ in = A.in;
+  const double* __restrict__ tr_in = A.tr_in;
+  double* __restrict__ out = A.out;
+  double* __restrict__ tr_out = A.tr_out;
+  const Grid& G = A.G;
+  const SweepK& K = A.K;
+  const int ntr = A.ntr;
+  const unsigned pos = A.pos;
+  using WGm = WalkGeom<ORD>;
+  constexpr int hs = WGm::hs, HB = WGm::HB, R = WGm::R, SEG = WK_SEG, WX = WK_WX;
+  constexpr int MT = AWFL_MAX_TRACERS;
+  constexpr int NM = 1 + DIR;
+  constexpr int NA = NGLL;
+  extern __shared__ double sm[];
+  const int n = DIR == 1 ? G.ny : G.nz;
+  const bool per = G.bc[DIR] == AWFL_BC_PERIODIC;
+  const int ntx = (G.nx + WX - 1) / WX;
+  const unsigned tile = tile_index();
+  const int x0 = (int)(tile % (unsigned)ntx) * WX;
+  const int other = (int)(tile / (unsigned)ntx);   // y: the level; z: the row
+  const int kl = DIR == 1 ? other : 0;
+  const int jl = DIR == 2 ? other : 0;
+  const int64_t stride = DIR == 1 ? G.sy : G.sz;
+  const int64_t base0 = DIR == 1 ? (int64_t)(hs + kl) * G.sz + (int64_t)hs * G.sy + hs + x0
+                                 : (int64_t)hs * G.sz + (int64_t)(hs + jl) * G.sy + hs + x0;
+  const int64_t nc = (int64_t)G.nx * G.ny * G.nz;
+  const WalkLds Lo = walk_lds(ntr, R, DIR == 2);
+  const int tid = threadIdx.x;
+  const int lw = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int c = tid & 63;
+  asm volatile("" : "+v"(c));   // lane addresses formed in each part, not held across the walk
+  const bool lvalid = x0 + lw < G.nx;
+  double* const ln = sm + lw * Lo.size;
+  double* const rg = ln + Lo.ring;                 // state rows, row l at rg + l * R
+  double* const mr = rg + 5 * R;                   // mixing ratios
+  double* const ms = mr + ntr * R;                 // masses
+  double* const srcr = ms + ntr * R;               // z sources
+  double* const cy = ln + Lo.carry;
+  double* const ct_hi = cy + 16;                   // per tracer: high edge of the last cell
+  double* const ct_tf = ct_hi + ntr;               // flux of its face (unlimited)
+  double* const ct_fac = ct_tf + ntr;              // factor of the cell before it
+  double* const ct_tfl = ct_fac + ntr;             // limited flux of the face before it
+  double* const hd = ln + Lo.head;
+  double* const tl = ln + Lo.tail;
+  double* const h_tlo0 = hd + WH_TR;
+  double* const h_tf1 = h_tlo0 + ntr;
+  double* const h_fac1 = h_tf1 + ntr;
+  double* const h_tfl2 = h_fac1 + ntr;
+  double* const h_cell = h_tfl2 + ntr;             // [2][6 + ntr]: state, masses, source
+  // tail: F of faces 0 and n [2][5], their limiter directions [2], tracer fluxes [2][ntr],
+  // factors of cells 0 and n-1 [2][ntr], limited faces n-1, 0, n, 1, n-2 [5][ntr]
+  double* const t_F = tl;
+  double* const t_dir = tl + 10;
+  double* const t_tf = tl + 12;
+  double* const t_fac = t_tf + 2 * ntr;
+  double* const t_lim = t_fac + 2 * ntr;
+  auto slot = [](int a) { return (a + HB) % R; };   // a >= -HB
+  auto bg_dens = [&](int cell) {                    // background density of a cell clamped into the line
+    const int cc = cell < 0 ? 0 : (cell >= n ? n - 1 : cell);
+    return G.hyDensCells[hs + (DIR == 2 ? cc : kl)];
+  };
+  auto upwind_dir = [](double u) { return u > 0 ? -1 : (u < 0 ? 1 : 0); };
+  auto face_ref = [&](int f) {
+    FaceRef fr;
+    if constexpr (DIR == 1) {
+      fr.rt = G.prRT[kl];
+      fr.q5 = G.prT[kl];
+    } else {
+      fr.rt = fr.q5 = G.prF[f];
+    }
+    return fr;
+  };
+  auto face_bg = [&](int f) {                      // background pressure a z face subtracts (S:1628-1629)
+    return f < G.nz ? G.hyPressureGLL[f * NGLL + 0] : G.hyPressureGLL[(G.nz - 1) * NGLL + NGLL - 1];
+  };
+
+  // ring loads: cells a0 .. a0 + cnt - 1 of every line of the tile, lanes across x
+  auto stage = [&](int a0, int cnt) {
+    const int lt = tid % WX;
+    if (x0 + lt >= G.nx) return;
+    double* const rt = sm + lt * Lo.size + Lo.ring;
+    for (int k = tid / WX; k < cnt; k += 256 / WX) {
+      const int a = a0 + k;
+      bool zn;
+      const int src = wrap_axis(a, n, per, zn);
+      const int64_t o = base0 + lt + (int64_t)src * stride;
+      double v[5], q[MT];
+      sfor<5>([&](auto l) { v[l] = in[l * G.sl + o]; });
+      sfor<MT>([&](auto t) {
+        if (t < ntr) q[t] = tr_in[t * G.sl + o];
+      });
+      const double bg = G.hyDensCells[hs + (DIR == 2 ? src : kl)];
+      const int sl = slot(a);
+      sfor<5>([&](auto l) { rt[l * R + sl] = (zn && l == NM) ? 0.0 : v[l]; });
+      const double rr = rcp(v[idR] + bg);
+      sfor<MT>([&](auto t) {
+        if (t < ntr) {
+          rt[(5 + t) * R + sl] = q[t] * rr;
+          rt[(5 + ntr + t) * R + sl] = q[t];
+        }
+      });
+    }
+  };
+  // the state output of cell `cell` of this wave's line
+  auto put_state = [&](int cell, const double (&Fl)[5], const double (&Fr)[5], const double* st0, int mstride,
+                       double sa) {
+    int k = kl, j = jl;
+    if constexpr (DIR == 1) j = cell;
+    else k = cell;
+    const int64_t o = base0 + lw + (int64_t)cell * stride;
+    const int64_t ot = ((int64_t)k * G.ny + j) * G.nx + x0 + lw;
+    sfor<5>([&](auto l) {
+      double td = -(Fr[l] - Fl[l]) * K.rd;
+      if (DIR == 2 && l == idW) td = sa + td;
+      if (G.sim2d && l == idV) td = 0.0;
+      if constexpr (TEND) out[l * nc + ot] = td;
+      else out[l * G.sl + o] = st0[l * mstride] + K.dt * td;
+    });
+  };
+  // state and tracer output of cell `cell` (the tail's cells)
+  auto put = [&](int cell, const double (&Fl)[5], const double (&Fr)[5], const double* tfl, const double* tfr, int ts,
+                 const double* st0, const double* ms0, int mstride, double sa) {
+    int k = kl, j = jl;
+    if constexpr (DIR == 1) j = cell;
+    else k = cell;
+    const int64_t o = base0 + lw + (int64_t)cell * stride;
+    const int64_t ot = ((int64_t)k * G.ny + j) * G.nx + x0 + lw;
+    sfor<5>([&](auto l) {
+      double td = -(Fr[l] - Fl[l]) * K.rd;
+      if (DIR == 2 && l == idW) td = sa + td;
+      if (G.sim2d && l == idV) td = 0.0;
+      if constexpr (TEND) out[l * nc + ot] = td;
+      else out[l * G.sl + o] = st0[l * mstride] + K.dt * td;
+    });
+    for (int t = 0; t < ntr; t++) {
+      const double td = -(tfr[t * ts] - tfl[t * ts]) * K.rd;
+      if constexpr (TEND) tr_out[t * nc + ot] = td;
+      else tr_out[t * G.sl + o] = ms0[t * mstride] + K.dt * td;
+    }
+  };
+
+  const int S = (n + SEG - 1) / SEG;
+  if (mode == 0) {
+  if (lvalid && c == 0)   // what the first step's lane 0 reads of a previous step: zeros
+    for (int e = 0; e < 16 + 4 * ntr; e++) cy[e] = 0.0;
+  stage(-HB, R);
+  return;
+  }
+  if (mode == 2) {
+    stage(s * SEG + hs, SEG);
+    return;
+  }
+  if (mode == 1) {
+    const int c0 = s * SEG;
+    const int X = c0 + c;
+    const int cw = s == S - 1 ? n - 1 - c0 : SEG - 1;   // the lane whose cell the next step (or the tail) continues from
+    if (!lvalid) return;
+    // (a) cell X: state edges, the source, and the density and normal
+    // momentum levels the tracer transform reads
+    double eL[5], eR[5], sa;
+    double rl[NA][NGLL], nl[NA][NGLL];
+    {
+      const int Xc = X < n ? X : n - 1;
+      CellBg<NGLL> bg{};
+      if constexpr (DIR == 2) {
+        sfor<NGLL>([&](auto ii) {
+          bg.rG[ii] = G.hyDensGLL[Xc * NGLL + ii];
+          bg.rtG[ii] = G.hyDensThetaGLL[Xc * NGLL + ii];
+          bg.pG[ii] = G.hyPressureGLL[Xc * NGLL + ii];
+          bg.ptG[ii] = G.prG[Xc * NGLL + ii];
+        });
+        bg.wall_bot = !per && X == 0;
+        bg.wall_top = !per && X == n - 1;
+      } else {
+        bg.r = G.hyDensCells[hs + kl];
+        bg.rt = G.hyDensThetaCells[hs + kl];
+        bg.pt = G.prT[kl];
+        bg.wall_bot = bg.wall_top = false;
+      }
+      double st[5][ORD];
+      const int s0 = slot(X - hs);
+      sfor<ORD>([&](auto k) {
+        const int sk = s0 + k < R ? s0 + k : s0 + k - R;
+        sfor<5>([&](auto l) { st[l][k] = rg[l * R + sk]; });
+      });
+      cell_edges<ORD, NGLL, DIR>(st, bg, G.weno_s, G.weno_w, K, eL, eR, sa, [&](const auto& q) {
+        sfor<NA>([&](auto kt) {
+          sfor<NGLL>([&](auto ii) {
+            rl[kt][ii] = q[idR][kt][ii];
+            nl[kt][ii] = q[NM][kt][ii];
+          });
+        });
+      });
+    }
+    if constexpr (DIR == 2) srcr[slot(X)] = sa;
+    if (s == 0 && c == 0) sfor<5>([&](auto l) { hd[WH_EL0 + l] = eL[l]; });
+    // (b) face X from the right edge of cell X-1 (the next lower lane, or
+    // the carried cell c0-1; at the first step lane 0 takes its own left
+    // edge: face 0 is formed again by the tail)
+    double F[5], un, up;
+    int dX;
+    {
+      double L[5], Rr[5];
+      sfor<5>([&](auto l) {
+        const double sh = __shfl_up(eR[l], 1);
+        L[l] = c == 0 ? (s == 0 ? eL[l] : cy[l]) : sh;
+        Rr[l] = eL[l];
+      });
+      if (c == cw) sfor<5>([&](auto l) { cy[l] = eR[l]; });
+      riemann<DIR>(L, Rr, face_ref(X < n ? X : n), F, un);
+      if constexpr (DIR == 2) F[idW] -= face_bg(X < n ? X : n);
+      up = un > 0 ? F[0] * rcp(L[0]) : F[0] * rcp(Rr[0]);
+      dX = upwind_dir(0.5 * (L[NM] + Rr[NM]));   // the normal momentum average (S:907)
+    }
+    if (s == 0 && c == 1) hd[WH_DIR1] = (double)dX;
+    const int dm1i = __shfl_up(dX, 1);
+    const int dm1 = c == 0 ? (int)cy[15] : dm1i;   // direction of face X-1
+    if (c == cw) cy[15] = (double)dX;
+    // (c) the state of cell X-2 from faces X-2 and X-1 (the update trails
+    // the faces by two cells: the tracers' limiter needs the factor of the
+    // cell on either side of a face)
+    const bool upd = X >= 4 && X < n;
+    const int sc = slot(X - 2);
+    {
+      double F1[5], F2[5];
+      sfor<5>([&](auto l) {
+        const double a = __shfl_up(F[l], 1);
+        F1[l] = c == 0 ? cy[5 + l] : a;
+      });
+      if (c == cw) sfor<5>([&](auto l) { cy[5 + l] = F[l]; });
+      sfor<5>([&](auto l) {
+        const double a = __shfl_up(F1[l], 1);
+        F2[l] = c == 0 ? cy[10 + l] : a;
+      });
+      if (c == cw) sfor<5>([&](auto l) { cy[10 + l] = F1[l]; });
+      if (upd) {
+        put_state(X - 2, F2, F1, rg + sc, R, DIR == 2 ? srcr[sc] : 0.0);
+      } else if (s == 0 && (c == 2 || c == 3)) {
+        // cells 0 and 1 as they came in, and faces 1 and 2, for the tail
+        double* const hc = h_cell + (c - 2) * (6 + ntr);
+        sfor<5>([&](auto l) { hc[l] = rg[l * R + sc]; });
+        for (int t = 0; t < ntr; t++) hc[5 + t] = ms[t * R + sc];
+        hc[5 + ntr] = DIR == 2 ? srcr[sc] : 0.0;
+        if (c == 3) sfor<5>([&](auto l) {
+            hd[WH_F1 + l] = F2[l];
+            hd[WH_F2 + l] = F1[l];
+          });
+      }
+    }
+    // (d) per tracer: cell X's edges (the line kernel's transform), face X's
+    // flux upwinded with the state's normal velocity, the limiter factor of
+    // cell X-1 (T1), face X-1 limited by its upwind cell, and cell X-2's
+    // update.  Neighbouring lanes' values come by shuffles; lane 0 takes the
+    // previous step's last cell from the carry area, which lane cw then
+    // overwrites for the next step.
+    double rr0[NGLL];
+    sfor<NGLL>([&](auto ii) { rr0[ii] = rcp(rl[0][ii]); });
+    const int sm1 = slot(X - 1);
+    const double dns = pos ? rg[idR * R + sm1] + bg_dens(X - 1) : 0.0;
+    int k = kl, j = jl;
+    if constexpr (DIR == 1) j = X - 2;
+    else k = X - 2;
+    const int64_t o2 = base0 + lw + (int64_t)(X - 2) * stride;
+    const int64_t ot2 = ((int64_t)k * G.ny + j) * G.nx + x0 + lw;
+    for (int t = 0; t < ntr; t++) {
+      const bool tp = (pos >> t) & 1u;
+      double u[ORD], g[NGLL];
+      const int s1 = slot(X - hs);
+      sfor<ORD>([&](auto kk) {
+        const int sk = s1 + kk < R ? s1 + kk : s1 + kk - R;
+        u[kk] = mr[t * R + sk];
+      });
+      reconstruct<ORD, NGLL>(u, G.weno_s, g);
+      double rt[NA][NGLL], rnt[NA][NGLL];
+      sfor<NGLL>([&](auto ii) {
+        rt[0][ii] = g[ii] * rl[0][ii];
+        if (tp) rt[0][ii] = fmax(0.0, rt[0][ii]);
+        rnt[0][ii] = rt[0][ii] * nl[0][ii] * rr0[ii];
+      });
+      // diffTransformTracer (S:2095-2126)
+      sfor<NA - 1>([&](auto kt) {
+        sfor<NGLL>([&](auto ii) {
+          double df = 0.0;
+          sfor<NGLL>([&](auto s2) { df += Deriv<NGLL>::m[s2][ii] * rnt[kt][s2]; });
+          rt[kt + 1][ii] = df * K.ndt[kt];
+        });
+        if constexpr (kt + 2 < NA) {
+          sfor<NGLL>([&](auto ii) {
+            double tot = 0.0;
+            sfor<kt + 2>([&](auto ir) {
+              if constexpr (ir == 0) tot += nl[ir][ii] * rt[kt + 1 - ir][ii];
+              else tot += nl[ir][ii] * rt[kt + 1 - ir][ii] - rl[ir][ii] * rnt[kt + 1 - ir][ii];
+            });
+            rnt[kt + 1][ii] = tot * rr0[ii];
+          });
+        }
+      });
+      double a0 = rt[0][0], a1 = rt[0][NGLL - 1];
+      sfor<NA - 1>([&](auto km) {
+        constexpr int kt = km + 1;
+        a0 += rt[kt][0] * K.tav[kt];
+        a1 += rt[kt][NGLL - 1] * K.tav[kt];
+      });
+      if (tp) {
+        a0 = fmax(0.0, a0);
+        a1 = fmax(0.0, a1);
+      }
+      if (s == 0 && c == 0) h_tlo0[t] = a0;
+      // face X: rho*q of the upwind side, the high edge of cell X-1 or the low edge of cell X
+      const double a1s = __shfl_up(a1, 1);
+      const double a1m = c == 0 ? ct_hi[t] : a1s;
+      const double tfX = (un > 0 ? a1m : a0) * up;
+      if (s == 0 && c == 1) h_tf1[t] = tfX;
+      const double tfs = __shfl_up(tfX, 1);
+      const double tfm = c == 0 ? ct_tf[t] : tfs;   // face X-1, unlimited
+      double lim = tfm, fv = 0.0;
+      if (tp) {
+        const double f1 = fmin(tfm, 0.0);
+        const double f2 = fmax(tfX, 0.0);
+        const double fluxOut = K.dt * (f2 - f1) * K.rd;
+        const double q = mr[t * R + sm1];
+        fv = fmin(1.0, q * dns * rcp(fluxOut + TRACER_EPS));   // factor of cell X-1
+        if (s == 0 && c == 2) h_fac1[t] = fv;
+        const double fs = __shfl_up(fv, 1);
+        const double fm = c == 0 ? ct_fac[t] : fs;   // factor of cell X-2
+        if (dm1 != 0) lim *= dm1 < 0 ? fm : fv;
+      }
+      if (s == 0 && c == 3) h_tfl2[t] = lim;
+      const double ls = __shfl_up(lim, 1);
+      const double lm = c == 0 ? ct_tfl[t] : ls;     // face X-2, limited
+      if (upd) {
+        const double td = -(lim - lm) * K.rd;
+        if constexpr (TEND) tr_out[t * nc + ot2] = td;
+        else tr_out[t * G.sl + o2] = ms[t * R + sc] + K.dt * td;
+      }
+      if (c == cw) {
+        ct_hi[t] = a1;
+        ct_tf[t] = tfX;
+        ct_fac[t] = fv;
+        ct_tfl[t] = lim;
+      }
+    }
+    return;
+  }
+  if (!lvalid) return;
+  wave_sync();
+
+  // tail: faces 0 (lane 0) and n (lane 1).  A periodic y line's face n is its
+  // face 0 (both lanes form the same flux); a periodic z line keeps both,
+  // which differ by the background pressure each subtracts.
+  if (c < 2) {
+    const bool f0 = c == 0;
+    const double* const eRn = cy;                  // right edge of cell n-1
+    const double* const eL0 = hd + WH_EL0;         // left edge of cell 0
+    double L[5], Rr[5], F[5], un;
+    sfor<5>([&](auto l) {
+      L[l] = (f0 && !per) ? eL0[l] : eRn[l];
+      Rr[l] = (!f0 && !per) ? eRn[l] : eL0[l];
+    });
+    riemann<DIR>(L, Rr, face_ref(f0 ? 0 : n), F, un);
+    if constexpr (DIR == 2) F[idW] -= face_bg(f0 ? 0 : n);
+    sfor<5>([&](auto l) { t_F[c * 5 + l] = F[l]; });
+    t_dir[c] = (double)upwind_dir(0.5 * (L[NM] + Rr[NM]));
+    const double up = un > 0 ? F[0] * rcp(L[0]) : F[0] * rcp(Rr[0]);
+    for (int t = 0; t < ntr; t++) {
+      const double thn = ct_hi[t];                 // high edge of cell n-1
+      const double tl0 = h_tlo0[t];                // low edge of cell 0
+      const double tL = (f0 && !per) ? tl0 : thn;
+      const double tR = (!f0 && !per) ? thn : tl0;
+      t_tf[c * ntr + t] = (un > 0 ? tL : tR) * up;
+    }
+  }
+  wave_sync();
+  // factors of cell 0 (lane 0: faces 0 and 1) and cell n-1 (lane 1: faces n-1 and n)
+  if (pos && c < 2) {
+    const bool f0 = c == 0;
+    const int sn = slot(n - 1);
+    const double* const hc0 = h_cell;
+    const double dns = f0 ? hc0[idR] + bg_dens(0) : rg[idR * R + sn] + bg_dens(n - 1);
+    const int o_l = f0 ? Lo.tail + 12 : Lo.carry + 16 + ntr;
+    const int o_r = f0 ? Lo.head + WH_TR + ntr : Lo.tail + 12 + ntr;
+    for (int t = 0; t < ntr; t++) {
+      if ((pos >> t) & 1u) {
+        const double fl = ln[o_l + t];
+        const double fr = ln[o_r + t];
+        const double f1 = fmin(fl, 0.0);
+        const double f2 = fmax(fr, 0.0);
+        const double fluxOut = K.dt * (f2 - f1) * K.rd;
+        const double q = f0 ? ln[Lo.head + WH_TR + 4 * ntr + 5 + t] * rcp(dns) : mr[t * R + sn];
+        t_fac[c * ntr + t] = fmin(1.0, q * dns * rcp(fluxOut + TRACER_EPS));
+      }
+    }
+  }
+  wave_sync();
+  // limited faces n-1, 0, n, 1 (lanes 0..3): per lane, the LDS offsets of
+  // the face's unlimited flux, its direction and the factors of the cells
+  // on its two sides, chosen as numbers (a chain of branches over lanes
+  // loading different addresses was compiled with one lane's address
+  // taken for another's)
+  if (c < 4) {
+    const int LT = Lo.tail;
+    const int o_tf = c == 0 ? Lo.carry + 16 + ntr : c == 1 ? LT + 12 : c == 2 ? LT + 12 + ntr : Lo.head + WH_TR + ntr;
+    const int o_d = c == 0 ? Lo.carry + 15 : c == 1 ? LT + 10 : c == 2 ? LT + 11 : Lo.head + WH_DIR1;
+    const int o_fl = c == 0 ? Lo.carry + 16 + 2 * ntr : c == 3 ? LT + 12 + 2 * ntr : LT + 12 + 3 * ntr;
+    const int o_fr = c == 0 ? LT + 12 + 3 * ntr : c == 3 ? Lo.head + WH_TR + 2 * ntr : LT + 12 + 2 * ntr;
+    const double d = ln[o_d];
+    for (int t = 0; t < ntr; t++) {
+      double v = ln[o_tf + t];
+      if (((pos >> t) & 1u) && d != 0.0) v *= d < 0 ? ln[o_fl + t] : ln[o_fr + t];
+      t_lim[c * ntr + t] = v;
+      if (c == 0) t_lim[4 * ntr + t] = ct_tfl[t];    // face n-2, limited by the last step
+    }
+  }
+  wave_sync();
+  // cells n-2, n-1, 0, 1 (lanes 0..3), their inputs chosen by LDS offset
+  if (c < 4) {
+    const int sc = slot(n - 2 + (c & 1));
+    const int o_hc = Lo.head + WH_TR + 4 * ntr + (c - 2) * (6 + ntr);   // c >= 2: the cell kept by the first step
+    // state fluxes of the left and right faces: F(n-2) | F(n-1) | F(n) | F(0) | F(1) | F(2)
+    const int o_Fl = c == 0 ? Lo.carry + 10 : c == 1 ? Lo.carry + 5 : c == 2 ? Lo.tail : Lo.head + WH_F1;
+    const int o_Fr = c == 0 ? Lo.carry + 5 : c == 1 ? Lo.tail + 5 : c == 2 ? Lo.head + WH_F1 : Lo.head + WH_F2;
+    double Fl[5], Fr[5];
+    sfor<5>([&](auto l) {
+      Fl[l] = ln[o_Fl + l];
+      Fr[l] = ln[o_Fr + l];
+    });
+    // the cell as it came in: the ring (cells n-2, n-1) or the head (cells 0, 1)
+    const int o_st = c < 2 ? Lo.ring + sc : o_hc;
+    const int mst = c < 2 ? R : 1;
+    const int o_ms = c < 2 ? Lo.ring + (5 + ntr) * R + sc : o_hc + 5;
+    const double sa = DIR == 2 ? ln[c < 2 ? Lo.ring + (5 + 2 * ntr) * R + sc : o_hc + 5 + ntr] : 0.0;
+    // limited fluxes of the cell's faces: n-2|n-1, n-1|n, 0|1, 1|2
+    const int o_tl = Lo.tail + 12 + 4 * ntr;
+    const int o_l = c == 0 ? o_tl + 4 * ntr : c == 1 ? o_tl : c == 2 ? o_tl + ntr : o_tl + 3 * ntr;
+    const int o_r = c == 0 ? o_tl : c == 1 ? o_tl + 2 * ntr : c == 2 ? o_tl + 3 * ntr : Lo.head + WH_TR + 3 * ntr;
+    put(c < 2 ? n - 2 + c : c - 2, Fl, Fr, ln + o_l, ln + o_r, 1, ln + o_st, ln + o_ms, mst, sa);
+  }
+}
+
+template <int ORD, int NGLL, int DIR, bool TEND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_tr(WalkArgs) {
+  const int n = DIR == 1 ? walk_args().G.ny : walk_args().G.nz;
+  const int S = (n + WK_SEG - 1) / WK_SEG;
+  walk<ORD, NGLL, DIR, TEND>(walk_args(), 0, 0);
+  __syncthreads();
+  for (int s = 0; s < S; s++) {
+    walk<ORD, NGLL, DIR, TEND>(walk_args(), 1, s);
+    if (s + 1 < S) {
+      __syncthreads();
+      walk<ORD, NGLL, DIR, TEND>(walk_args(), 2, s + 1);
+      __syncthreads();
+    }
+  }
+  walk<ORD, NGLL, DIR, TEND>(walk_args(), 3, S - 1);
+}
+
 // tracers += dt * tracer_tend over the interior (Temporal_ader.h:67-69)
 __global__ void k_apply_tr(double* __restrict__ tr, const double* __restrict__ ttend, Grid G, int ntr, double dt) {
   const int64_t nc = (int64_t)G.nx * G.ny * G.nz;
@@ -468,21 +1035,29 @@ __global__ void k_fill_tr(double* __restrict__ p, int64_t n, double v) {
 }
 
 using KernelTr = void (*)(const double*, const double*, double*, double*, Grid, SweepK, int, unsigned);
+using KernelWk = void (*)(WalkArgs);
 
 struct VariantTr {
   int ord, ngll;
   KernelTr upd[3], tend[3];
+  KernelWk wupd[2], wtend[2];   // y and z walkers (null: the line kernels run y and z)
   void (*init)(double*, Grid, int, double, double);
 };
 
-#define AWFL_VARIANT_TR(O, N)                                                                  \
+#define AWFL_VARIANT_TR(O, N, WU, WT)                                                          \
   VariantTr {                                                                                  \
     O, N, {k_sweep_tr<O, N, 0, false>, k_sweep_tr<O, N, 1, false>, k_sweep_tr_z<O, N, false>}, \
-        {k_sweep_tr<O, N, 0, true>, k_sweep_tr<O, N, 1, true>, k_sweep_tr_z<O, N, true>},       \
+        {k_sweep_tr<O, N, 0, true>, k_sweep_tr<O, N, 1, true>, k_sweep_tr_z<O, N, true>}, WU, WT, \
         k_init_tr<O>,                                                                          \
   }
+#define AWFL_WALK_TR(O, N, T) {k_walk_tr<O, N, 1, T>, k_walk_tr<O, N, 2, T>}
 
-const VariantTr kVariantsTr[] = {AWFL_VARIANT_TR(3, 2), AWFL_VARIANT_TR(5, 3), AWFL_VARIANT_TR(9, 5)};
+// (9, 5) keeps the line kernels in y and z: its cell is too large for a
+// work-group of four lines
+const VariantTr kVariantsTr[] = {
+    AWFL_VARIANT_TR(3, 2, AWFL_WALK_TR(3, 2, false), AWFL_WALK_TR(3, 2, true)),
+    AWFL_VARIANT_TR(5, 3, AWFL_WALK_TR(5, 3, false), AWFL_WALK_TR(5, 3, true)),
+    AWFL_VARIANT_TR(9, 5, {}, {})};
 
 const VariantTr* find_variant_tr(int ord, int ngll) {
   for (const VariantTr& v : kVariantsTr)
@@ -501,12 +1076,26 @@ int launch_sweep_tr(awfl_ctx* c, const double* in, const double* tr_in, double* 
   if (!v) return fail(c, AWFL_E_UNSUPPORTED, "tracers: (ord, ngll) not compiled in");
   const int n = dir == 0 ? c->cfg.nx : dir == 1 ? c->cfg.ny : c->cfg.nz;
   const int W = n + 2 * c->hs, ntr = c->ntr;
+  const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
+  if (dir > 0 && n >= 8 && v->wupd[dir - 1]) {
+    // the walker: tiles of WK_WX x-adjacent lines
+    const int hs = c->hs, HB = hs > 2 ? hs : 2;
+    const WalkLds L = walk_lds(ntr, WK_SEG + HB + hs, dir == 2);
+    const size_t lds = sizeof(double) * (size_t)L.size * WK_WX;
+    KernelWk k = tend ? v->wtend[dir - 1] : v->wupd[dir - 1];
+    if (lds > 64 * 1024)
+      HIP_TRY(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const unsigned tiles = (unsigned)((c->cfg.nx + WK_WX - 1) / WK_WX) * (unsigned)(dir == 1 ? c->cfg.nz : c->cfg.ny);
+    const WalkArgs a{in, tr_in, out, tr_out, c->grid(), sweep_constants(c, dt, dir), ntr, c->tpos};
+    k<<<tiles, 64 * WK_WX, lds, c->stream>>>(a);
+    LAUNCH_CHECK(c);
+    return AWFL_OK;
+  }
   const size_t lds = sizeof(double) * (size_t)(5 * W + ntr * W + 10 * n + 2 * ntr * n + (dir == 2 ? 2 : 1) * n);
   if (lds > 160 * 1024) return fail(c, AWFL_E_UNSUPPORTED, "tracers: line too long for the tracer sweep's LDS");
   KernelTr k = tend ? v->tend[dir] : v->upd[dir];
   if (lds > 64 * 1024)
     HIP_TRY(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int64_t nc = (int64_t)c->cfg.nx * c->cfg.ny * c->cfg.nz;
   const unsigned lines = (unsigned)(nc / n);
   const int threads = line_threads(n);
   k<<<lines, threads, lds, c->stream>>>(in, tr_in, out, tr_out, c->grid(), sweep_constants(c, dt, dir), ntr,
