@@ -458,6 +458,7 @@ struct WalkLds {
   int ring;   // rows of R: state [5], mixing ratio [ntr], mass [ntr], (z) source [1]
   int carry;  // of the step's last cell X: right edge [5], F(X) [5], F(X-1) [5], limiter direction [1],
               // then per tracer its high edge, flux of face X, factor of cell X-1, limited flux of face X-1
+  int outb;   // [5 + ntr][SEG] the step's outputs (cells c0-2 .. c0+61), stored with the lanes across x
   int head;   // cells 0 and 1 (first step), for the tail
   int tail;   // faces 0 and n, factors of cells 0 and n-1, limited faces n-1, 0, n, 1, n-2
   int size;
@@ -469,6 +470,7 @@ __host__ __device__ inline WalkLds walk_lds(int ntr, int R, bool z) {
   int o = 0;
   L.ring = o;  o += (5 + 2 * ntr + (z ? 1 : 0)) * R;
   L.carry = o; o += 16 + 4 * ntr;
+  L.outb = o;  o += (5 + ntr) * WK_SEG;
   L.head = o;  o += WH_TR + 4 * ntr + 2 * (6 + ntr);
   L.tail = o;  o += 12 + 9 * ntr;
   L.size = o;
@@ -520,7 +522,8 @@ __device__ __forceinline__ WalkArgs walk_args() {
 }
 
 // one part of the walk: 0 the first step's ring and the carries, 1 step s,
-// 2 the ring loads of step s, 3 the tail
+// 2 the outputs of step s-1 and the ring loads of step s, 3 the tail, 4 the
+// outputs of step s
 template <int ORD, int NGLL, int DIR, bool TEND>
 __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
   static_assert(DIR == 1 || DIR == 2, "the walker runs the y and z sweeps");
@@ -561,6 +564,7 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
   double* const mr = rg + 5 * R;                   // mixing ratios
   double* const ms = mr + ntr * R;                 // masses
   double* const srcr = ms + ntr * R;               // z sources
+  double* const ob = ln + Lo.outb;
   double* const cy = ln + Lo.carry;
   double* const ct_hi = cy + 16;                   // per tracer: high edge of the last cell
   double* const ct_tf = ct_hi + ntr;               // flux of its face (unlimited)
@@ -627,22 +631,6 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
       });
     }
   };
-  // the state output of cell `cell` of this wave's line
-  auto put_state = [&](int cell, const double (&Fl)[5], const double (&Fr)[5], const double* st0, int mstride,
-                       double sa) {
-    int k = kl, j = jl;
-    if constexpr (DIR == 1) j = cell;
-    else k = cell;
-    const int64_t o = base0 + lw + (int64_t)cell * stride;
-    const int64_t ot = ((int64_t)k * G.ny + j) * G.nx + x0 + lw;
-    sfor<5>([&](auto l) {
-      double td = -(Fr[l] - Fl[l]) * K.rd;
-      if (DIR == 2 && l == idW) td = sa + td;
-      if (G.sim2d && l == idV) td = 0.0;
-      if constexpr (TEND) out[l * nc + ot] = td;
-      else out[l * G.sl + o] = st0[l * mstride] + K.dt * td;
-    });
-  };
   // state and tracer output of cell `cell` (the tail's cells)
   auto put = [&](int cell, const double (&Fl)[5], const double (&Fr)[5], const double* tfl, const double* tfr, int ts,
                  const double* st0, const double* ms0, int mstride, double sa) {
@@ -672,8 +660,34 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
   stage(-HB, R);
   return;
   }
+  // the outputs of step sf, lanes across x: cell X-2 for X = sf*SEG + k
+  auto flush = [&](int sf) {
+    const int lt = tid % WX, k = tid / WX;
+    const int X = sf * SEG + k;
+    if (x0 + lt >= G.nx || X < 4 || X >= n) return;
+    const double* const obt = sm + lt * Lo.size + Lo.outb;
+    const int cell = X - 2;
+    int kk = kl, j = jl;
+    if constexpr (DIR == 1) j = cell;
+    else kk = cell;
+    const int64_t o = base0 + lt + (int64_t)cell * stride;
+    const int64_t ot = ((int64_t)kk * G.ny + j) * G.nx + x0 + lt;
+    sfor<5>([&](auto l) {
+      if constexpr (TEND) out[l * nc + ot] = obt[l * SEG + k];
+      else out[l * G.sl + o] = obt[l * SEG + k];
+    });
+    for (int t = 0; t < ntr; t++) {
+      if constexpr (TEND) tr_out[t * nc + ot] = obt[(5 + t) * SEG + k];
+      else tr_out[t * G.sl + o] = obt[(5 + t) * SEG + k];
+    }
+  };
   if (mode == 2) {
+    flush(s - 1);
     stage(s * SEG + hs, SEG);
+    return;
+  }
+  if (mode == 4) {
+    flush(s);
     return;
   }
   if (mode == 1) {
@@ -760,7 +774,13 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
       });
       if (c == cw) sfor<5>([&](auto l) { cy[10 + l] = F1[l]; });
       if (upd) {
-        put_state(X - 2, F2, F1, rg + sc, R, DIR == 2 ? srcr[sc] : 0.0);
+        const double sa2 = DIR == 2 ? srcr[sc] : 0.0;
+        sfor<5>([&](auto l) {
+          double td = -(F1[l] - F2[l]) * K.rd;
+          if (DIR == 2 && l == idW) td = sa2 + td;
+          if (G.sim2d && l == idV) td = 0.0;
+          ob[l * SEG + c] = TEND ? td : rg[l * R + sc] + K.dt * td;
+        });
       } else if (s == 0 && (c == 2 || c == 3)) {
         // cells 0 and 1 as they came in, and faces 1 and 2, for the tail
         double* const hc = h_cell + (c - 2) * (6 + ntr);
@@ -783,11 +803,6 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
     sfor<NGLL>([&](auto ii) { rr0[ii] = rcp(rl[0][ii]); });
     const int sm1 = slot(X - 1);
     const double dns = pos ? rg[idR * R + sm1] + bg_dens(X - 1) : 0.0;
-    int k = kl, j = jl;
-    if constexpr (DIR == 1) j = X - 2;
-    else k = X - 2;
-    const int64_t o2 = base0 + lw + (int64_t)(X - 2) * stride;
-    const int64_t ot2 = ((int64_t)k * G.ny + j) * G.nx + x0 + lw;
     for (int t = 0; t < ntr; t++) {
       const bool tp = (pos >> t) & 1u;
       double u[ORD], g[NGLL];
@@ -856,8 +871,7 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
       const double lm = c == 0 ? ct_tfl[t] : ls;     // face X-2, limited
       if (upd) {
         const double td = -(lim - lm) * K.rd;
-        if constexpr (TEND) tr_out[t * nc + ot2] = td;
-        else tr_out[t * G.sl + o2] = ms[t * R + sc] + K.dt * td;
+        ob[(5 + t) * SEG + c] = TEND ? td : ms[t * R + sc] + K.dt * td;
       }
       if (c == cw) {
         ct_hi[t] = a1;
@@ -977,6 +991,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       __syncthreads();
     }
   }
+  __syncthreads();
+  walk<ORD, NGLL, DIR, TEND>(walk_args(), 4, S - 1);
   walk<ORD, NGLL, DIR, TEND>(walk_args(), 3, S - 1);
 }
 
