@@ -605,11 +605,16 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
   };
 
   // ring loads: cells a0 .. a0 + cnt - 1 of every line of the tile, lanes across x
-  auto stage = [&](int a0, int cnt) {
+  // mid: between the loads and the LDS stores (the previous step's outputs
+  // are stored there, while the loads are in flight)
+  auto stage = [&](int a0, int cnt, auto&& mid) {
     const int lt = tid % WX;
-    if (x0 + lt >= G.nx) return;
+    if (x0 + lt >= G.nx) {
+      mid();
+      return;
+    }
     double* const rt = sm + lt * Lo.size + Lo.ring;
-    for (int k = tid / WX; k < cnt; k += 256 / WX) {
+    for (int k = tid / WX; k < cnt; k += SEG) {
       const int a = a0 + k;
       bool zn;
       const int src = wrap_axis(a, n, per, zn);
@@ -620,6 +625,7 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
         if (t < ntr) q[t] = tr_in[t * G.sl + o];
       });
       const double bg = G.hyDensCells[hs + (DIR == 2 ? src : kl)];
+      if (k == tid / WX) mid();
       const int sl = slot(a);
       sfor<5>([&](auto l) { rt[l * R + sl] = (zn && l == NM) ? 0.0 : v[l]; });
       const double rr = rcp(v[idR] + bg);
@@ -657,7 +663,7 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
   if (mode == 0) {
   if (lvalid && c == 0)   // what the first step's lane 0 reads of a previous step: zeros
     for (int e = 0; e < 16 + 4 * ntr; e++) cy[e] = 0.0;
-  stage(-HB, R);
+  stage(-HB, R, [] {});
   return;
   }
   // the outputs of step sf, lanes across x: cell X-2 for X = sf*SEG + k
@@ -682,8 +688,7 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
     }
   };
   if (mode == 2) {
-    flush(s - 1);
-    stage(s * SEG + hs, SEG);
+    stage(s * SEG + hs, SEG, [&] { flush(s - 1); });
     return;
   }
   if (mode == 4) {
