@@ -460,7 +460,8 @@ struct WalkLds {
               // then per tracer its high edge, flux of face X, factor of cell X-1, limited flux of face X-1
   int outb;   // [5 + ntr][SEG] the step's outputs (cells c0-2 .. c0+61), stored with the lanes across x
   int head;   // cells 0 and 1 (first step), for the tail
-  int tail;   // faces 0 and n, factors of cells 0 and n-1, limited faces n-1, 0, n, 1, n-2
+  int tail;   // faces 0 and n, factors of cells 0 and n-1, limited faces n-1, 0, n, 1, n-2; then
+              // the constants the tail reads (loaded by the first part of the walk)
   int size;
 };
 // head offsets
@@ -472,7 +473,7 @@ __host__ __device__ inline WalkLds walk_lds(int ntr, int R, bool z) {
   L.carry = o; o += 16 + 4 * ntr;
   L.outb = o;  o += (5 + ntr) * WK_SEG;
   L.head = o;  o += WH_TR + 4 * ntr + 2 * (6 + ntr);
-  L.tail = o;  o += 12 + 9 * ntr;
+  L.tail = o;  o += 12 + 9 * ntr + 20;
   L.size = o;
   return L;
 }
@@ -584,6 +585,9 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
   double* const t_tf = tl + 12;
   double* const t_fac = t_tf + 2 * ntr;
   double* const t_lim = t_fac + 2 * ntr;
+  // tail constants: power references of faces 0 and n [2][8], the background
+  // pressure each subtracts (z) [2], background densities of cells 0 and n-1 [2]
+  double* const t_con = t_lim + 5 * ntr;
   auto slot = [](int a) { return (a + HB) % R; };   // a >= -HB
   auto bg_dens = [&](int cell) {                    // background density of a cell clamped into the line
     const int cc = cell < 0 ? 0 : (cell >= n ? n - 1 : cell);
@@ -661,8 +665,22 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
 
   const int S = (n + SEG - 1) / SEG;
   if (mode == 0) {
-  if (lvalid && c == 0)   // what the first step's lane 0 reads of a previous step: zeros
+  if (lvalid && c == 0) {   // what the first step's lane 0 reads of a previous step: zeros
     for (int e = 0; e < 16 + 4 * ntr; e++) cy[e] = 0.0;
+    for (int f = 0; f < 2; f++) {
+      const FaceRef fr = face_ref(f == 0 ? 0 : n);
+      const PowRef* pr[2] = {&fr.rt, &fr.q5};
+      for (int r = 0; r < 2; r++) {
+        t_con[f * 8 + r * 4 + 0] = pr[r]->h;
+        t_con[f * 8 + r * 4 + 1] = pr[r]->l;
+        t_con[f * 8 + r * 4 + 2] = pr[r]->rb;
+        t_con[f * 8 + r * 4 + 3] = pr[r]->b;
+      }
+      t_con[16 + f] = DIR == 2 ? face_bg(f == 0 ? 0 : n) : 0.0;
+    }
+    t_con[18] = bg_dens(0);
+    t_con[19] = bg_dens(n - 1);
+  }
   stage(-HB, R, [] {});
   return;
   }
@@ -902,8 +920,12 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
       L[l] = (f0 && !per) ? eL0[l] : eRn[l];
       Rr[l] = (!f0 && !per) ? eRn[l] : eL0[l];
     });
-    riemann<DIR>(L, Rr, face_ref(f0 ? 0 : n), F, un);
-    if constexpr (DIR == 2) F[idW] -= face_bg(f0 ? 0 : n);
+    FaceRef fr;
+    const double* const tc = t_con + c * 8;
+    fr.rt = PowRef{tc[0], tc[1], tc[2], tc[3]};
+    fr.q5 = PowRef{tc[4], tc[5], tc[6], tc[7]};
+    riemann<DIR>(L, Rr, fr, F, un);
+    if constexpr (DIR == 2) F[idW] -= t_con[16 + c];
     sfor<5>([&](auto l) { t_F[c * 5 + l] = F[l]; });
     t_dir[c] = (double)upwind_dir(0.5 * (L[NM] + Rr[NM]));
     const double up = un > 0 ? F[0] * rcp(L[0]) : F[0] * rcp(Rr[0]);
@@ -921,7 +943,7 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
     const bool f0 = c == 0;
     const int sn = slot(n - 1);
     const double* const hc0 = h_cell;
-    const double dns = f0 ? hc0[idR] + bg_dens(0) : rg[idR * R + sn] + bg_dens(n - 1);
+    const double dns = f0 ? hc0[idR] + t_con[18] : rg[idR * R + sn] + t_con[19];
     const int o_l = f0 ? Lo.tail + 12 : Lo.carry + 16 + ntr;
     const int o_r = f0 ? Lo.head + WH_TR + ntr : Lo.tail + 12 + ntr;
     for (int t = 0; t < ntr; t++) {
@@ -996,9 +1018,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       __syncthreads();
     }
   }
+  walk<ORD, NGLL, DIR, TEND>(walk_args(), 3, S - 1);   // the tail: this wave's line only
   __syncthreads();
   walk<ORD, NGLL, DIR, TEND>(walk_args(), 4, S - 1);
-  walk<ORD, NGLL, DIR, TEND>(walk_args(), 3, S - 1);
 }
 
 // tracers += dt * tracer_tend over the interior (Temporal_ader.h:67-69)
