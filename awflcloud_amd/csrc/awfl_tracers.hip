@@ -20,10 +20,12 @@
 //   a positivity flux limiter for positive tracers (S:904-935);
 //   the flux-difference tendency, and the tracer masses multiplied back
 //   (S:948-955).
-// Here one work-group runs one line along the sweep direction with the line
-// in LDS, state and tracers together; the state part runs the production
-// sweep's statements (cell_edges / riemann), so the state it returns is
-// awfl_sweep's up to the compiler's contraction choices in this kernel.
+// Here the x sweep runs one line per work-group with the line in LDS, state
+// and tracers together (k_sweep_tr), and the y and z sweeps run the walker
+// (k_walk_tr: four x-adjacent lines per work-group walked in 64-cell steps);
+// both run the production sweep's statements (cell_edges / riemann) for the
+// state, so the state they return is awfl_sweep's up to the compiler's
+// contraction choices in these kernels.
 // Departures from the source text, where it cannot run as written:
 //   T1 the limiter forms every cell's reduction factor from the unlimited
 //      fluxes and then applies it to the faces it is upwind of; the
