@@ -131,6 +131,46 @@ def test_tendencies_match_oracle(case, direction):
     assert np.abs(I(got) - I(upd)).max() <= 4e-16 * np.abs(I(upd)).max()
 
 
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_eight_tracers_match_oracle(direction):
+    """Eight tracers (AWFL_MAX_TRACERS), three of them not positive, over
+    several walker steps with a partial tile of lines: the per-tracer loops
+    and carries beyond the driver's three, and the walker's launch with more
+    than 64 KB of LDS (82 KB in z)."""
+    _need_gpu()
+    from awflcloud_amd import Dycore, bubble_spec
+    n, bc = (10, 72, 68), (0, 0, 1)
+    pos = (0, 1, 1, 0, 1, 1, 0, 1)
+    nx, ny, nz = n
+    D = Dycore(nx, ny, nz, bc=bc)
+    st = D.init_state(bubble_spec(bubble_dicts(), **_spec_kw(ny)))
+    dt = D.compute_time_step(st, 0.8)
+    D.set_tracers(list(pos))
+    O = P.Oracle(nx, ny, nz, bc=bc)
+    O.set_tracers(pos)
+    O.set_dt_cache(dt)
+    host = st.cpu().numpy().copy()
+    rho = O.density(host)
+    f3 = _tracer_fields(nx, ny, nz, O.dx, O.dy, O.dz, rho)
+    tr_host = np.zeros(O.tracer_shape())
+    extra = [0.5 * f3[1] - 0.1 * rho, 0.3 * f3[2], f3[1] + f3[2], 0.2 * rho, 2.0 * f3[1]]
+    O.tracer_interior(tr_host)[...] = np.concatenate([f3, extra])
+    tr = torch.from_numpy(tr_host.copy()).cuda()
+    print("eight tracers, dir", direction, flush=True)
+    tend = D.new_tend()
+    ttend = torch.zeros(D.tracer_tend_shape, dtype=torch.float64, device="cuda")
+    D.dim_switch = 1
+    D.compute_tendencies_tracers(st, tr, tend, ttend, dt, direction)
+    otend, ottend = O.tendencies_dir_tr(host, tr_host, dt, direction)
+    g = ttend.cpu().numpy()
+    scale = np.array([max(np.abs(ottend[t]).max(), 1e-300) for t in range(len(pos))])
+    err = _tr_err(g, ottend, scale)
+    serr = float(np.abs(tend.cpu().numpy() - otend).max() / np.abs(otend).max())
+    print("tracer tendency err %.2e  state tendency err %.2e" % (err, serr), flush=True)
+    assert err <= 5e-11
+    assert serr <= 1e-11
+
+
 @pytest.mark.parametrize("case", [CASES[i] for i in STEP_CASES], ids=[IDS[i] for i in STEP_CASES])
 def test_time_steps_match_oracle(case):
     _need_gpu()
