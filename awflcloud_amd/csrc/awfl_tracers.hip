@@ -758,6 +758,9 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
       });
     }
     if constexpr (DIR == 2) srcr[slot(X)] = sa;
+    // lane X-2 reads this source in the update below; the fence also ends the
+    // cell's schedule here (measured: y 0.803 -> 0.786 ms, its spill halved)
+    wave_sync();
     if (s == 0 && c == 0) sfor<5>([&](auto l) { hd[WH_EL0 + l] = eL[l]; });
     // (b) face X from the right edge of cell X-1 (the next lower lane, or
     // the carried cell c0-1; at the first step lane 0 takes its own left
