@@ -435,7 +435,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 //   and a face's factor from its upwind cell, so a lane forms the factor of
 //   cell X - 1, limits face X - 1 and updates cell X - 2: the update trails
 //   the faces by two cells.  What the next step's first lanes need of the
-//   last cells of this one is kept in a small carry area.
+//   last cells of this one is kept in a small carry area.  Per tracer the
+//   edges, fluxes, factors and limited fluxes pass between lanes by
+//   shuffles, in one loop.
+// - Outputs go to an LDS buffer and are stored with the lanes across x at
+//   the next step boundary, while that step's ring loads are in flight.
 // - Cells 0 and 1 need face 0, which a periodic line forms from the last
 //   cell's edge (and the limiter of either boundary may take the factor of
 //   the cell at the other end, as the line kernel's indexing does), so the
@@ -667,24 +671,24 @@ __device__ __forceinline__ void walk(const WalkArgs& A, int mode, int s) {
 
   const int S = (n + SEG - 1) / SEG;
   if (mode == 0) {
-  if (lvalid && c == 0) {   // what the first step's lane 0 reads of a previous step: zeros
-    for (int e = 0; e < 16 + 4 * ntr; e++) cy[e] = 0.0;
-    for (int f = 0; f < 2; f++) {
-      const FaceRef fr = face_ref(f == 0 ? 0 : n);
-      const PowRef* pr[2] = {&fr.rt, &fr.q5};
-      for (int r = 0; r < 2; r++) {
-        t_con[f * 8 + r * 4 + 0] = pr[r]->h;
-        t_con[f * 8 + r * 4 + 1] = pr[r]->l;
-        t_con[f * 8 + r * 4 + 2] = pr[r]->rb;
-        t_con[f * 8 + r * 4 + 3] = pr[r]->b;
+    if (lvalid && c == 0) {   // zeros for what step 0's lane 0 reads of a previous step; the tail's constants
+      for (int e = 0; e < 16 + 4 * ntr; e++) cy[e] = 0.0;
+      for (int f = 0; f < 2; f++) {
+        const FaceRef fr = face_ref(f == 0 ? 0 : n);
+        const PowRef* pr[2] = {&fr.rt, &fr.q5};
+        for (int r = 0; r < 2; r++) {
+          t_con[f * 8 + r * 4 + 0] = pr[r]->h;
+          t_con[f * 8 + r * 4 + 1] = pr[r]->l;
+          t_con[f * 8 + r * 4 + 2] = pr[r]->rb;
+          t_con[f * 8 + r * 4 + 3] = pr[r]->b;
+        }
+        t_con[16 + f] = DIR == 2 ? face_bg(f == 0 ? 0 : n) : 0.0;
       }
-      t_con[16 + f] = DIR == 2 ? face_bg(f == 0 ? 0 : n) : 0.0;
+      t_con[18] = bg_dens(0);
+      t_con[19] = bg_dens(n - 1);
     }
-    t_con[18] = bg_dens(0);
-    t_con[19] = bg_dens(n - 1);
-  }
-  stage(-HB, R, [] {});
-  return;
+    stage(-HB, R, [] {});
+    return;
   }
   // the outputs of step sf, lanes across x: cell X-2 for X = sf*SEG + k
   auto flush = [&](int sf) {
